@@ -1,0 +1,140 @@
+/*
+ * remora_hip/capi.h -- C-ABI of the MI355X (gfx950) dense-Cholesky library.
+ *
+ * Replaces the layer under Remora's hip_tag kernels: the cuBLAS-backed
+ * hip::blas_device (include/remora/hip/cublas_backend.hpp:86-450) and the host
+ * recursion of kernels::potrf on hip_tag (include/remora/kernels/hip/potrf.hpp:83-140).
+ *
+ * Conventions (kept from blas_device, cublas_backend.hpp:121-431):
+ *   - BLAS column-major storage; element (i,j) of a matrix with leading
+ *     dimension ld is at ptr[i + j*ld].  Row-major callers flip orientation
+ *     the way Remora's glue does (include/remora/kernels/hip/{gemm,syrk,trsm,
+ *     trsv}.hpp).
+ *   - bool-like int flags, 64-bit sizes (the reference narrowed to int for
+ *     cuBLAS, cublas_backend.hpp:136-141; these do not).
+ *   - every call is asynchronous on the given stream; nothing synchronises
+ *     unless the function says so.  A NULL stream is the default stream.
+ *   - device pointers are caller-owned; no exception crosses the ABI: each
+ *     function returns an rmh_status, and rmh_last_error() gives the detail
+ *     of the last failure on the calling thread.
+ *   - a handle owns scratch memory and must be used by one host thread /
+ *     stream at a time (like a cuBLAS handle); create one per thread.
+ */
+#ifndef REMORA_HIP_CAPI_H
+#define REMORA_HIP_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+struct ihipStream_t;                      /* hipStream_t without the HIP headers */
+typedef struct ihipStream_t* rmh_stream_t;
+typedef struct rmh_context* rmh_handle;
+
+typedef enum rmh_status {
+    RMH_STATUS_SUCCESS = 0,
+    RMH_STATUS_INVALID_VALUE = 1,          /* bad size, leading dimension, flag or NULL pointer */
+    RMH_STATUS_INVALID_HANDLE = 2,
+    RMH_STATUS_HIP_ERROR = 3,              /* a HIP runtime call or kernel launch failed */
+    RMH_STATUS_ALLOC_FAILED = 4,           /* scratch memory could not be allocated */
+    RMH_STATUS_NOT_POSITIVE_DEFINITE = 5,  /* reported by synchronous entry points only */
+    RMH_STATUS_NOT_CONVERGED = 6,          /* iterative refinement reached max_iter */
+    RMH_STATUS_NOT_SUPPORTED = 7
+} rmh_status;
+
+/* ---------------------------------------------------------- lifetime -- */
+/* One handle per device per host thread: replaces the thread-local
+ * blas_device cache of get_blas (cublas_backend.hpp:442-450). */
+rmh_status rmh_create(rmh_handle* handle, int device);
+rmh_status rmh_destroy(rmh_handle handle);
+const char* rmh_status_string(rmh_status status);
+/* Detail of the last non-success status returned on the calling thread. */
+const char* rmh_last_error(void);
+
+/* ------------------------------------------------------------- BLAS-3 -- */
+/* C = alpha*op(A)*op(B) + beta*C; replaces blas_device::gemm
+ * (cublas_backend.hpp:121-166).  beta == 0 does not read C. */
+rmh_status rmh_dgemm(rmh_handle h, int transA, int transB, size_t m, size_t n, size_t k, double alpha,
+                     const double* A, size_t lda, const double* B, size_t ldb, double beta, double* C,
+                     size_t ldc, rmh_stream_t stream);
+rmh_status rmh_sgemm(rmh_handle h, int transA, int transB, size_t m, size_t n, size_t k, float alpha,
+                     const float* A, size_t lda, const float* B, size_t ldb, float beta, float* C, size_t ldc,
+                     rmh_stream_t stream);
+
+/* C = alpha*op(A)*op(A)^T + beta*C on the upper or lower triangle only;
+ * op(A) = A (n x k) if !trans, A^T (A is k x n) if trans.  Replaces
+ * blas_device::syrk (cublas_backend.hpp:389-431). */
+rmh_status rmh_dsyrk(rmh_handle h, int upper, int trans, size_t n, size_t k, double alpha, const double* A,
+                     size_t lda, double beta, double* C, size_t ldc, rmh_stream_t stream);
+rmh_status rmh_ssyrk(rmh_handle h, int upper, int trans, size_t n, size_t k, float alpha, const float* A,
+                     size_t lda, float beta, float* C, size_t ldc, rmh_stream_t stream);
+
+/* Solve op(A) X = alpha B (left) or X op(A) = alpha B (right) in place in B
+ * (m x n); A is triangular of order m (left) or n (right).  Replaces
+ * blas_device::trsm (cublas_backend.hpp:303-344).  Like the cuBLAS routine it
+ * replaces, a zero on the diagonal is not detected (the result holds inf/nan). */
+rmh_status rmh_dtrsm(rmh_handle h, int left, int upper, int trans, int unit, size_t m, size_t n, double alpha,
+                     const double* A, size_t lda, double* B, size_t ldb, rmh_stream_t stream);
+rmh_status rmh_strsm(rmh_handle h, int left, int upper, int trans, int unit, size_t m, size_t n, float alpha,
+                     const float* A, size_t lda, float* B, size_t ldb, rmh_stream_t stream);
+
+/* ------------------------------------------------------------- BLAS-2 -- */
+/* Solve op(A) x = b in place (x has stride incx).  Replaces
+ * blas_device::trsv (cublas_backend.hpp:348-383). */
+rmh_status rmh_dtrsv(rmh_handle h, int upper, int trans, int unit, size_t n, const double* A, size_t lda,
+                     double* x, size_t incx, rmh_stream_t stream);
+rmh_status rmh_strsv(rmh_handle h, int upper, int trans, int unit, size_t n, const float* A, size_t lda,
+                     float* x, size_t incx, rmh_stream_t stream);
+
+/* y = alpha*op(A)*x + beta*y, A is m x n.  Replaces blas_device::gemv
+ * (cublas_backend.hpp:170-213). */
+rmh_status rmh_dgemv(rmh_handle h, int trans, size_t m, size_t n, double alpha, const double* A, size_t lda,
+                     const double* x, size_t incx, double beta, double* y, size_t incy, rmh_stream_t stream);
+rmh_status rmh_sgemv(rmh_handle h, int trans, size_t m, size_t n, float alpha, const float* A, size_t lda,
+                     const float* x, size_t incx, float beta, float* y, size_t incy, rmh_stream_t stream);
+
+/* ------------------------------------------------------------ LAPACK -- */
+/* In-place Cholesky factorisation A = L L^T (lower) or U^T U (upper); only
+ * that triangle is read and written.  Replaces the host recursion of
+ * kernels::potrf on hip_tag (include/remora/kernels/hip/potrf.hpp:83-140) with
+ * a device-side blocked right-looking factorisation.
+ * d_info (device int64, may be NULL) receives LAPACK's info: 0 on success,
+ * k+1 if the leading minor of order k+1 is not positive definite (the
+ * factorisation stops there).  Asynchronous: read d_info after the stream. */
+rmh_status rmh_dpotrf(rmh_handle h, int upper, size_t n, double* A, size_t lda, int64_t* d_info,
+                      rmh_stream_t stream);
+rmh_status rmh_spotrf(rmh_handle h, int upper, size_t n, float* A, size_t lda, int64_t* d_info,
+                      rmh_stream_t stream);
+
+/* Solve A X = B with the factor from rmh_?potrf (two triangular solves, as
+ * cholesky_decomposition::solve does, include/remora/decompositions.hpp:101-105). */
+rmh_status rmh_dpotrs(rmh_handle h, int upper, size_t n, size_t nrhs, const double* A, size_t lda, double* B,
+                      size_t ldb, rmh_stream_t stream);
+rmh_status rmh_spotrs(rmh_handle h, int upper, size_t n, size_t nrhs, const float* A, size_t lda, float* B,
+                      size_t ldb, rmh_stream_t stream);
+
+/* ------------------------------------------------------------ tuning -- */
+/* Block sizes of rmh_?potrf: nb is the width of a panel (the K of the
+ * trailing update), nb_inner the diagonal block inside a panel (<= 64).
+ * 0 restores the defaults.  Applies to later calls on this handle. */
+rmh_status rmh_set_potrf_blocking(rmh_handle h, size_t nb, size_t nb_inner);
+
+/* ---------------------------------------------------------- profiling -- */
+/* Kernel classes: bit 0 "trailing_update", bit 1 "panel", bit 2 "diag",
+ * bit 3 "inner_update".  While enabled, the handle records HIP events around
+ * every launch of a class in class_mask on the stream it is launched on and
+ * accumulates the algorithmic flops of each launch (class_mask 0 disables,
+ * -1 enables all).  rmh_profile_read waits for the recorded events and
+ * returns totals since the last reset. */
+rmh_status rmh_profile_enable(rmh_handle h, int class_mask);
+rmh_status rmh_profile_reset(rmh_handle h);
+rmh_status rmh_profile_read(rmh_handle h, const char* kernel_class, double* total_ms, double* total_flops,
+                            int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* REMORA_HIP_CAPI_H */

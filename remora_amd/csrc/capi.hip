@@ -1,0 +1,584 @@
+// C-ABI entry points of libremora_hip and the host-side drivers behind them:
+// handle lifetime, BLAS-3/2 front ends, the blocked TRSM, and the
+// device-side blocked right-looking POTRF with its solve.
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "rmh_context.hpp"
+
+namespace rmh {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+rmh_status fail(rmh_status st, const std::string& msg) {
+    set_error(msg);
+    return st;
+}
+
+rmh_status check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(RMH_STATUS_HIP_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+    return RMH_STATUS_SUCCESS;
+}
+
+// ------------------------------------------------------------ context --
+rmh_status Context::activate() {
+    RMH_TRY_HIP(hipSetDevice(device));
+    return RMH_STATUS_SUCCESS;
+}
+
+rmh_status Context::reserve(size_t bytes) {
+    if (bytes <= ws_bytes) return RMH_STATUS_SUCCESS;
+    // Kernels queued earlier may still use the old buffer.
+    RMH_TRY_HIP(hipDeviceSynchronize());
+    if (ws != nullptr) RMH_TRY_HIP(hipFree(ws));
+    ws = nullptr;
+    ws_bytes = 0;
+    size_t want = std::max(bytes, (size_t)1 << 20);
+    if (hipMalloc(&ws, want) != hipSuccess) {
+        (void)hipGetLastError();
+        ws = nullptr;
+        return fail(RMH_STATUS_ALLOC_FAILED, "workspace allocation of " + std::to_string(want) + " bytes failed");
+    }
+    ws_bytes = want;
+    return RMH_STATUS_SUCCESS;
+}
+
+rmh_status Context::prof_begin(hipStream_t s, int cls, double flops, int* slot) {
+    *slot = -1;
+    if (((prof_mask >> cls) & 1) == 0) return RMH_STATUS_SUCCESS;
+    ProfRec r;
+    r.cls = cls;
+    r.flops = flops;
+    RMH_TRY_HIP(hipEventCreate(&r.a));
+    RMH_TRY_HIP(hipEventCreate(&r.b));
+    RMH_TRY_HIP(hipEventRecord(r.a, s));
+    recs.push_back(r);
+    *slot = (int)recs.size() - 1;
+    return RMH_STATUS_SUCCESS;
+}
+
+rmh_status Context::prof_end(hipStream_t s, int slot) {
+    if (slot < 0) return RMH_STATUS_SUCCESS;
+    RMH_TRY_HIP(hipEventRecord(recs[slot].b, s));
+    return RMH_STATUS_SUCCESS;
+}
+
+void Context::prof_clear() {
+    for (auto& r : recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    recs.clear();
+}
+
+Context::~Context() {
+    prof_clear();
+    if (ws != nullptr) {
+        (void)hipSetDevice(device);
+        (void)hipFree(ws);
+    }
+}
+
+// ------------------------------------------------------- TRSM driver --
+// Blocked recursive solve with inverted 64 x 64 diagonal blocks: the leaves
+// multiply by the inverse block (in place, LDS-staged), the off-diagonal
+// work is MFMA GEMM.  The split mirrors the reference's trsm_recursive
+// (include/remora/kernels/default/trsm.hpp:227-261): halves in whole
+// blocks, lower solves front-to-back, upper back-to-front.
+namespace {
+
+constexpr int kTrsmNb = 64;
+
+template <typename T>
+struct TrsmPlan {
+    bool left, eff_upper, trans;
+    const T* A;
+    int64_t lda;
+    T* B;
+    int64_t ldb;
+    int64_t na, other;
+    const T* Dinv;
+    hipStream_t s;
+};
+
+// Block T[r0:r1, c0:c1] of op(A) as a GEMM operand.
+template <typename T>
+void tri_operand(const TrsmPlan<T>& p, int64_t r0, int64_t c0, const T** ptr, bool* tr) {
+    if (!p.trans) { *ptr = p.A + r0 + c0 * p.lda; *tr = false; }
+    else { *ptr = p.A + c0 + r0 * p.lda; *tr = true; }
+}
+
+template <typename T>
+rmh_status trsm_rec(const TrsmPlan<T>& p, int64_t b0, int64_t b1) {
+    const int64_t nb = kTrsmNb;
+    if (b1 - b0 == 1) {
+        const int kb = (int)std::min(nb, p.na - b0 * nb);
+        return launch_tri_apply<T>(p.left, p.Dinv + b0 * nb * nb, nb, kb, p.B, p.ldb, b0 * nb, p.other, p.s);
+    }
+    const int64_t mid = b0 + (b1 - b0) / 2;
+    const int64_t r0 = b0 * nb, rm = mid * nb, r1 = std::min(b1 * nb, p.na);
+    GemmDesc<T> g;
+    g.alpha = T(-1);
+    g.beta = T(1);
+    if (p.left) {
+        // forward (eff lower): B[rm:r1] -= T[rm:r1, r0:rm] X[r0:rm]
+        // backward (eff upper): B[r0:rm] -= T[r0:rm, rm:r1] X[rm:r1]
+        const bool fwd = !p.eff_upper;
+        const int64_t cr0 = fwd ? rm : r0, cr1 = fwd ? r1 : rm;  // rows of C
+        const int64_t xr0 = fwd ? r0 : rm, xr1 = fwd ? rm : r1;  // rows of X
+        if (fwd) RMH_TRY(trsm_rec(p, b0, mid));
+        else RMH_TRY(trsm_rec(p, mid, b1));
+        g.M = cr1 - cr0; g.N = p.other; g.K = xr1 - xr0;
+        tri_operand(p, cr0, xr0, &g.A, &g.transA);
+        g.lda = p.lda;
+        g.B = p.B + xr0; g.ldb = p.ldb; g.transB = false;
+        g.C = p.B + cr0; g.ldc = p.ldb;
+        RMH_TRY(launch_gemm<T>(g, pick_tile(g.M, g.N, Part::Full), p.s));
+        if (fwd) return trsm_rec(p, mid, b1);
+        return trsm_rec(p, b0, mid);
+    }
+    // right side, X T = B: eff lower runs back-to-front, eff upper front-to-back.
+    //   eff lower: B[:, r0:rm] -= X[:, rm:r1] T[rm:r1, r0:rm]
+    //   eff upper: B[:, rm:r1] -= X[:, r0:rm] T[r0:rm, rm:r1]
+    const bool fwd = p.eff_upper;
+    const int64_t cc0 = fwd ? rm : r0, cc1 = fwd ? r1 : rm;  // columns of C
+    const int64_t xc0 = fwd ? r0 : rm, xc1 = fwd ? rm : r1;  // columns of X
+    if (fwd) RMH_TRY(trsm_rec(p, b0, mid));
+    else RMH_TRY(trsm_rec(p, mid, b1));
+    g.M = p.other; g.N = cc1 - cc0; g.K = xc1 - xc0;
+    g.A = p.B + xc0 * p.ldb; g.lda = p.ldb; g.transA = false;
+    tri_operand(p, xc0, cc0, &g.B, &g.transB);
+    g.ldb = p.lda;
+    g.C = p.B + cc0 * p.ldb; g.ldc = p.ldb;
+    RMH_TRY(launch_gemm<T>(g, pick_tile(g.M, g.N, Part::Full), p.s));
+    if (fwd) return trsm_rec(p, mid, b1);
+    return trsm_rec(p, b0, mid);
+}
+
+}  // namespace
+
+template <typename T>
+rmh_status trsm_impl(Context* h, bool left, bool upper, bool trans, bool unit, int64_t m, int64_t n, T alpha,
+                     const T* A, int64_t lda, T* B, int64_t ldb, hipStream_t s) {
+    if (m == 0 || n == 0) return RMH_STATUS_SUCCESS;
+    if (alpha != T(1)) RMH_TRY(launch_scale<T>(B, ldb, m, n, alpha, s));
+    if (alpha == T(0)) return RMH_STATUS_SUCCESS;
+    const int64_t na = left ? m : n;
+    const int64_t nblk = (na + kTrsmNb - 1) / kTrsmNb;
+    RMH_TRY(h->reserve((size_t)nblk * kTrsmNb * kTrsmNb * sizeof(T)));
+    T* Dinv = static_cast<T*>(h->ws);
+    RMH_TRY(launch_trtri_blocks<T>(A, lda, na, kTrsmNb, upper, trans, unit, Dinv, s));
+    TrsmPlan<T> p{left, upper != trans, trans, A, lda, B, ldb, na, left ? n : m, Dinv, s};
+    return trsm_rec(p, 0, nblk);
+}
+
+// ------------------------------------------------------ POTRF driver --
+// Blocked right-looking Cholesky on the lower view V of A: V(i,j) is
+// A[i + j*lda] for uplo = lower and A[j + i*lda] for uplo = upper (so the
+// upper factor U is computed as the lower factor of A^T, as the reference's
+// potrf_recursive does through trans(), include/remora/kernels/default/potrf.hpp:136-145).
+//
+// For each panel of width nb:
+//   for each diagonal block of width nbi inside the panel:
+//     diag   -- factor V_jj in LDS and form W = V_jj^-1 (one wavefront)
+//     panel  -- V_ij <- V_ij W^T for the rows below, in place (MFMA GEMM)
+//     inner  -- rank-nbi update of the rest of the panel's columns
+//   trailing -- V22 -= P P^T with K = nb (MFMA SYRK on the lower trapezoid)
+// Every launch reads the device info word first and does nothing once a
+// pivot has failed.
+template <typename T>
+rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int64_t* d_info, hipStream_t s) {
+    const int64_t nb = (int64_t)h->nb, nbi = (int64_t)h->nbi;
+    const size_t w_off = 256;
+    RMH_TRY(h->reserve(w_off + (size_t)nbi * nbi * sizeof(T)));
+    int64_t* info = d_info != nullptr ? d_info : static_cast<int64_t*>(h->ws);
+    T* W = reinterpret_cast<T*>(static_cast<char*>(h->ws) + w_off);
+    RMH_TRY(launch_fill_i64(info, 1, 0, s));
+    if (n == 0) return RMH_STATUS_SUCCESS;
+    const int64_t rs = upper ? lda : 1, cs = upper ? 1 : lda;
+    auto V = [&](int64_t i, int64_t j) { return A + i * rs + j * cs; };
+
+    for (int64_t k0 = 0; k0 < n; k0 += nb) {
+        const int64_t kw = std::min(nb, n - k0);
+        for (int64_t j0 = k0; j0 < k0 + kw; j0 += nbi) {
+            const int64_t jb = std::min(nbi, k0 + kw - j0);
+            int slot;
+            RMH_TRY(h->prof_begin(s, PROF_DIAG, (double)jb * jb * jb / 3.0, &slot));
+            RMH_TRY(launch_potrf_diag<T>(V(j0, j0), rs, cs, (int)jb, W, nbi, info, j0, s));
+            RMH_TRY(h->prof_end(s, slot));
+            const int64_t below = n - j0 - jb;
+            if (below > 0) {
+                // panel solve V21 <- V21 W^T, in place
+                GemmDesc<T> g;
+                g.alpha = T(1); g.beta = T(0); g.abort_flag = info;
+                Tile tile;
+                if (!upper) {
+                    g.M = below; g.N = jb; g.K = jb;
+                    g.A = A + (j0 + jb) + j0 * lda; g.lda = lda; g.transA = false;
+                    g.B = W; g.ldb = nbi; g.transB = true;
+                    g.C = A + (j0 + jb) + j0 * lda; g.ldc = lda;
+                    tile = below >= 128 * 256 ? Tile::T128x64 : Tile::T64x64;
+                } else {  // A12 <- W A12 (jb x below)
+                    g.M = jb; g.N = below; g.K = jb;
+                    g.A = W; g.lda = nbi; g.transA = false;
+                    g.B = A + j0 + (j0 + jb) * lda; g.ldb = lda; g.transB = false;
+                    g.C = A + j0 + (j0 + jb) * lda; g.ldc = lda;
+                    tile = below >= 128 * 256 ? Tile::T64x128 : Tile::T64x64;
+                }
+                RMH_TRY(h->prof_begin(s, PROF_PANEL, 2.0 * below * jb * jb / 2.0, &slot));
+                RMH_TRY(launch_gemm<T>(g, tile, s));
+                RMH_TRY(h->prof_end(s, slot));
+            }
+            const int64_t cw = k0 + kw - (j0 + jb);
+            if (cw > 0) {
+                // inner update of the remaining panel columns
+                GemmDesc<T> g;
+                g.alpha = T(-1); g.beta = T(1); g.abort_flag = info; g.K = jb;
+                if (!upper) {
+                    g.M = below; g.N = cw; g.part = Part::Lower;
+                    g.A = A + (j0 + jb) + j0 * lda; g.lda = lda; g.transA = false;
+                    g.B = g.A; g.ldb = lda; g.transB = true;
+                    g.C = A + (j0 + jb) + (j0 + jb) * lda; g.ldc = lda;
+                } else {
+                    g.M = cw; g.N = below; g.part = Part::Upper;
+                    g.A = A + j0 + (j0 + jb) * lda; g.lda = lda; g.transA = true;
+                    g.B = g.A; g.ldb = lda; g.transB = false;
+                    g.C = A + (j0 + jb) + (j0 + jb) * lda; g.ldc = lda;
+                }
+                RMH_TRY(h->prof_begin(s, PROF_INNER, 2.0 * jb * ((double)cw * (cw + 1) / 2.0 + (double)(below - cw) * cw), &slot));
+                RMH_TRY(launch_gemm<T>(g, Tile::T64x64, s));
+                RMH_TRY(h->prof_end(s, slot));
+            }
+        }
+        const int64_t rt = n - k0 - kw;
+        if (rt > 0) {
+            GemmDesc<T> g;
+            g.alpha = T(-1); g.beta = T(1); g.abort_flag = info;
+            g.M = rt; g.N = rt; g.K = kw;
+            if (!upper) {
+                g.part = Part::Lower;
+                g.A = A + (k0 + kw) + k0 * lda; g.lda = lda; g.transA = false;
+                g.B = g.A; g.ldb = lda; g.transB = true;
+            } else {
+                g.part = Part::Upper;
+                g.A = A + k0 + (k0 + kw) * lda; g.lda = lda; g.transA = true;
+                g.B = g.A; g.ldb = lda; g.transB = false;
+            }
+            g.C = A + (k0 + kw) + (k0 + kw) * lda; g.ldc = lda;
+            int slot;
+            RMH_TRY(h->prof_begin(s, PROF_TRAILING, (double)rt * (rt + 1) * kw, &slot));
+            RMH_TRY(launch_gemm<T>(g, pick_tile(rt, rt, g.part), s));
+            RMH_TRY(h->prof_end(s, slot));
+        }
+    }
+    return RMH_STATUS_SUCCESS;
+}
+
+template <typename T>
+rmh_status potrs_impl(Context* h, bool upper, int64_t n, int64_t nrhs, const T* A, int64_t lda, T* B, int64_t ldb,
+                      hipStream_t s) {
+    if (n == 0 || nrhs == 0) return RMH_STATUS_SUCCESS;
+    // lower: L y = b, L^T x = y ; upper: U^T y = b, U x = y
+    RMH_TRY(trsm_impl<T>(h, true, upper, upper, false, n, nrhs, T(1), A, lda, B, ldb, s));
+    return trsm_impl<T>(h, true, upper, !upper, false, n, nrhs, T(1), A, lda, B, ldb, s);
+}
+
+template <typename T>
+rmh_status trsv_impl(Context* h, bool upper, bool trans, bool unit, int64_t n, const T* A, int64_t lda, T* x,
+                     int64_t incx, hipStream_t s) {
+    if (n == 0) return RMH_STATUS_SUCCESS;
+    if (incx == 1) return trsm_impl<T>(h, true, upper, trans, unit, n, 1, T(1), A, lda, x, n, s);
+    // gather a strided vector into the tail of the workspace, solve, scatter
+    const int64_t nblk = (n + kTrsmNb - 1) / kTrsmNb;
+    const size_t dinv_bytes = (size_t)nblk * kTrsmNb * kTrsmNb * sizeof(T);
+    const size_t off = (dinv_bytes + 255) / 256 * 256;
+    RMH_TRY(h->reserve(off + (size_t)n * sizeof(T)));
+    T* tmp = reinterpret_cast<T*>(static_cast<char*>(h->ws) + off);
+    RMH_TRY(launch_strided_copy<T>(x, incx, tmp, 1, n, s));
+    RMH_TRY(trsm_impl<T>(h, true, upper, trans, unit, n, 1, T(1), A, lda, tmp, n, s));
+    return launch_strided_copy<T>(tmp, 1, x, incx, n, s);
+}
+
+}  // namespace rmh
+
+using namespace rmh;
+
+// ------------------------------------------------------- validation ---
+namespace {
+
+Context* ctx(rmh_handle h) { return reinterpret_cast<Context*>(h); }
+
+#define RMH_CHECK_HANDLE(h)                                                          \
+    do {                                                                             \
+        if ((h) == nullptr) return fail(RMH_STATUS_INVALID_HANDLE, "NULL handle");   \
+        RMH_TRY(ctx(h)->activate());                                                 \
+    } while (0)
+
+#define RMH_REQUIRE(cond, msg)                                                       \
+    do {                                                                             \
+        if (!(cond)) return fail(RMH_STATUS_INVALID_VALUE, (msg));                   \
+    } while (0)
+
+inline size_t mx1(size_t v) { return v > 1 ? v : 1; }
+
+template <typename T>
+rmh_status gemm_entry(rmh_handle h, int transA, int transB, size_t m, size_t n, size_t k, T alpha, const T* A,
+                      size_t lda, const T* B, size_t ldb, T beta, T* C, size_t ldc, rmh_stream_t stream) {
+    RMH_CHECK_HANDLE(h);
+    RMH_REQUIRE(lda >= mx1(transA ? k : m), "gemm: lda too small");
+    RMH_REQUIRE(ldb >= mx1(transB ? n : k), "gemm: ldb too small");
+    RMH_REQUIRE(ldc >= mx1(m), "gemm: ldc too small");
+    if (m == 0 || n == 0) return RMH_STATUS_SUCCESS;
+    RMH_REQUIRE(C != nullptr, "gemm: C is NULL");
+    RMH_REQUIRE(k == 0 || alpha == T(0) || (A != nullptr && B != nullptr), "gemm: A or B is NULL");
+    GemmDesc<T> g;
+    g.M = (int64_t)m; g.N = (int64_t)n; g.K = (alpha == T(0)) ? 0 : (int64_t)k;
+    g.alpha = alpha; g.beta = beta;
+    g.A = A; g.lda = (int64_t)lda; g.transA = transA != 0;
+    g.B = B; g.ldb = (int64_t)ldb; g.transB = transB != 0;
+    g.C = C; g.ldc = (int64_t)ldc;
+    return launch_gemm<T>(g, pick_tile(g.M, g.N, Part::Full), hs(stream));
+}
+
+template <typename T>
+rmh_status syrk_entry(rmh_handle h, int upper, int trans, size_t n, size_t k, T alpha, const T* A, size_t lda,
+                      T beta, T* C, size_t ldc, rmh_stream_t stream) {
+    RMH_CHECK_HANDLE(h);
+    RMH_REQUIRE(lda >= mx1(trans ? k : n), "syrk: lda too small");
+    RMH_REQUIRE(ldc >= mx1(n), "syrk: ldc too small");
+    if (n == 0) return RMH_STATUS_SUCCESS;
+    RMH_REQUIRE(C != nullptr, "syrk: C is NULL");
+    RMH_REQUIRE(k == 0 || alpha == T(0) || A != nullptr, "syrk: A is NULL");
+    GemmDesc<T> g;
+    g.M = (int64_t)n; g.N = (int64_t)n; g.K = (alpha == T(0)) ? 0 : (int64_t)k;
+    g.alpha = alpha; g.beta = beta;
+    g.part = upper ? Part::Upper : Part::Lower;
+    // C += alpha op(A) op(A)^T: op(A) = A (n x k) or A^T (A is k x n)
+    g.A = A; g.lda = (int64_t)lda; g.transA = trans != 0;
+    g.B = A; g.ldb = (int64_t)lda; g.transB = trans == 0;
+    g.C = C; g.ldc = (int64_t)ldc;
+    return launch_gemm<T>(g, pick_tile(g.M, g.N, g.part), hs(stream));
+}
+
+template <typename T>
+rmh_status trsm_entry(rmh_handle h, int left, int upper, int trans, int unit, size_t m, size_t n, T alpha,
+                      const T* A, size_t lda, T* B, size_t ldb, rmh_stream_t stream) {
+    RMH_CHECK_HANDLE(h);
+    const size_t na = left ? m : n;
+    RMH_REQUIRE(lda >= mx1(na), "trsm: lda too small");
+    RMH_REQUIRE(ldb >= mx1(m), "trsm: ldb too small");
+    if (m == 0 || n == 0) return RMH_STATUS_SUCCESS;
+    RMH_REQUIRE(B != nullptr && (A != nullptr || alpha == T(0)), "trsm: NULL pointer");
+    return trsm_impl<T>(ctx(h), left != 0, upper != 0, trans != 0, unit != 0, (int64_t)m, (int64_t)n, alpha, A,
+                        (int64_t)lda, B, (int64_t)ldb, hs(stream));
+}
+
+template <typename T>
+rmh_status trsv_entry(rmh_handle h, int upper, int trans, int unit, size_t n, const T* A, size_t lda, T* x,
+                      size_t incx, rmh_stream_t stream) {
+    RMH_CHECK_HANDLE(h);
+    RMH_REQUIRE(lda >= mx1(n), "trsv: lda too small");
+    RMH_REQUIRE(incx >= 1, "trsv: incx must be positive");
+    if (n == 0) return RMH_STATUS_SUCCESS;
+    RMH_REQUIRE(A != nullptr && x != nullptr, "trsv: NULL pointer");
+    return trsv_impl<T>(ctx(h), upper != 0, trans != 0, unit != 0, (int64_t)n, A, (int64_t)lda, x, (int64_t)incx,
+                        hs(stream));
+}
+
+template <typename T>
+rmh_status gemv_entry(rmh_handle h, int trans, size_t m, size_t n, T alpha, const T* A, size_t lda, const T* x,
+                      size_t incx, T beta, T* y, size_t incy, rmh_stream_t stream) {
+    RMH_CHECK_HANDLE(h);
+    RMH_REQUIRE(lda >= mx1(m), "gemv: lda too small");
+    RMH_REQUIRE(incx >= 1 && incy >= 1, "gemv: increments must be positive");
+    const size_t ylen = trans ? n : m;
+    if (ylen == 0) return RMH_STATUS_SUCCESS;
+    RMH_REQUIRE(y != nullptr, "gemv: y is NULL");
+    const size_t xlen = trans ? m : n;
+    RMH_REQUIRE(xlen == 0 || (A != nullptr && x != nullptr), "gemv: NULL pointer");
+    return launch_gemv<T>(trans != 0, (int64_t)m, (int64_t)n, alpha, A, (int64_t)lda, x, (int64_t)incx, beta, y,
+                          (int64_t)incy, hs(stream));
+}
+
+template <typename T>
+rmh_status potrf_entry(rmh_handle h, int upper, size_t n, T* A, size_t lda, int64_t* d_info, rmh_stream_t stream) {
+    RMH_CHECK_HANDLE(h);
+    RMH_REQUIRE(lda >= mx1(n), "potrf: lda too small");
+    RMH_REQUIRE(n == 0 || A != nullptr, "potrf: A is NULL");
+    return potrf_impl<T>(ctx(h), upper != 0, (int64_t)n, A, (int64_t)lda, d_info, hs(stream));
+}
+
+template <typename T>
+rmh_status potrs_entry(rmh_handle h, int upper, size_t n, size_t nrhs, const T* A, size_t lda, T* B, size_t ldb,
+                       rmh_stream_t stream) {
+    RMH_CHECK_HANDLE(h);
+    RMH_REQUIRE(lda >= mx1(n), "potrs: lda too small");
+    RMH_REQUIRE(ldb >= mx1(n), "potrs: ldb too small");
+    if (n == 0 || nrhs == 0) return RMH_STATUS_SUCCESS;
+    RMH_REQUIRE(A != nullptr && B != nullptr, "potrs: NULL pointer");
+    return potrs_impl<T>(ctx(h), upper != 0, (int64_t)n, (int64_t)nrhs, A, (int64_t)lda, B, (int64_t)ldb,
+                         hs(stream));
+}
+
+}  // namespace
+
+extern "C" {
+
+rmh_status rmh_create(rmh_handle* handle, int device) {
+    if (handle == nullptr) return fail(RMH_STATUS_INVALID_VALUE, "create: handle pointer is NULL");
+    *handle = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess) return fail(RMH_STATUS_HIP_ERROR, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+    if (device < 0 || device >= count)
+        return fail(RMH_STATUS_INVALID_VALUE, "create: device " + std::to_string(device) + " not present");
+    Context* c = new Context();
+    c->device = device;
+    rmh_status st = c->activate();
+    if (st != RMH_STATUS_SUCCESS) {
+        delete c;
+        return st;
+    }
+    *handle = reinterpret_cast<rmh_handle>(c);
+    return RMH_STATUS_SUCCESS;
+}
+
+rmh_status rmh_destroy(rmh_handle handle) {
+    if (handle == nullptr) return fail(RMH_STATUS_INVALID_HANDLE, "NULL handle");
+    delete ctx(handle);
+    return RMH_STATUS_SUCCESS;
+}
+
+const char* rmh_status_string(rmh_status status) {
+    switch (status) {
+        case RMH_STATUS_SUCCESS: return "success";
+        case RMH_STATUS_INVALID_VALUE: return "invalid value";
+        case RMH_STATUS_INVALID_HANDLE: return "invalid handle";
+        case RMH_STATUS_HIP_ERROR: return "HIP runtime error";
+        case RMH_STATUS_ALLOC_FAILED: return "allocation failed";
+        case RMH_STATUS_NOT_POSITIVE_DEFINITE: return "matrix is not positive definite";
+        case RMH_STATUS_NOT_CONVERGED: return "iterative refinement did not converge";
+        case RMH_STATUS_NOT_SUPPORTED: return "not supported";
+    }
+    return "unknown status";
+}
+
+const char* rmh_last_error(void) { return g_last_error.c_str(); }
+
+rmh_status rmh_dgemm(rmh_handle h, int transA, int transB, size_t m, size_t n, size_t k, double alpha,
+                     const double* A, size_t lda, const double* B, size_t ldb, double beta, double* C, size_t ldc,
+                     rmh_stream_t stream) {
+    return gemm_entry<double>(h, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream);
+}
+rmh_status rmh_sgemm(rmh_handle h, int transA, int transB, size_t m, size_t n, size_t k, float alpha,
+                     const float* A, size_t lda, const float* B, size_t ldb, float beta, float* C, size_t ldc,
+                     rmh_stream_t stream) {
+    return gemm_entry<float>(h, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream);
+}
+rmh_status rmh_dsyrk(rmh_handle h, int upper, int trans, size_t n, size_t k, double alpha, const double* A,
+                     size_t lda, double beta, double* C, size_t ldc, rmh_stream_t stream) {
+    return syrk_entry<double>(h, upper, trans, n, k, alpha, A, lda, beta, C, ldc, stream);
+}
+rmh_status rmh_ssyrk(rmh_handle h, int upper, int trans, size_t n, size_t k, float alpha, const float* A,
+                     size_t lda, float beta, float* C, size_t ldc, rmh_stream_t stream) {
+    return syrk_entry<float>(h, upper, trans, n, k, alpha, A, lda, beta, C, ldc, stream);
+}
+rmh_status rmh_dtrsm(rmh_handle h, int left, int upper, int trans, int unit, size_t m, size_t n, double alpha,
+                     const double* A, size_t lda, double* B, size_t ldb, rmh_stream_t stream) {
+    return trsm_entry<double>(h, left, upper, trans, unit, m, n, alpha, A, lda, B, ldb, stream);
+}
+rmh_status rmh_strsm(rmh_handle h, int left, int upper, int trans, int unit, size_t m, size_t n, float alpha,
+                     const float* A, size_t lda, float* B, size_t ldb, rmh_stream_t stream) {
+    return trsm_entry<float>(h, left, upper, trans, unit, m, n, alpha, A, lda, B, ldb, stream);
+}
+rmh_status rmh_dtrsv(rmh_handle h, int upper, int trans, int unit, size_t n, const double* A, size_t lda, double* x,
+                     size_t incx, rmh_stream_t stream) {
+    return trsv_entry<double>(h, upper, trans, unit, n, A, lda, x, incx, stream);
+}
+rmh_status rmh_strsv(rmh_handle h, int upper, int trans, int unit, size_t n, const float* A, size_t lda, float* x,
+                     size_t incx, rmh_stream_t stream) {
+    return trsv_entry<float>(h, upper, trans, unit, n, A, lda, x, incx, stream);
+}
+rmh_status rmh_dgemv(rmh_handle h, int trans, size_t m, size_t n, double alpha, const double* A, size_t lda,
+                     const double* x, size_t incx, double beta, double* y, size_t incy, rmh_stream_t stream) {
+    return gemv_entry<double>(h, trans, m, n, alpha, A, lda, x, incx, beta, y, incy, stream);
+}
+rmh_status rmh_sgemv(rmh_handle h, int trans, size_t m, size_t n, float alpha, const float* A, size_t lda,
+                     const float* x, size_t incx, float beta, float* y, size_t incy, rmh_stream_t stream) {
+    return gemv_entry<float>(h, trans, m, n, alpha, A, lda, x, incx, beta, y, incy, stream);
+}
+rmh_status rmh_dpotrf(rmh_handle h, int upper, size_t n, double* A, size_t lda, int64_t* d_info,
+                      rmh_stream_t stream) {
+    return potrf_entry<double>(h, upper, n, A, lda, d_info, stream);
+}
+rmh_status rmh_spotrf(rmh_handle h, int upper, size_t n, float* A, size_t lda, int64_t* d_info,
+                      rmh_stream_t stream) {
+    return potrf_entry<float>(h, upper, n, A, lda, d_info, stream);
+}
+rmh_status rmh_dpotrs(rmh_handle h, int upper, size_t n, size_t nrhs, const double* A, size_t lda, double* B,
+                      size_t ldb, rmh_stream_t stream) {
+    return potrs_entry<double>(h, upper, n, nrhs, A, lda, B, ldb, stream);
+}
+rmh_status rmh_spotrs(rmh_handle h, int upper, size_t n, size_t nrhs, const float* A, size_t lda, float* B,
+                      size_t ldb, rmh_stream_t stream) {
+    return potrs_entry<float>(h, upper, n, nrhs, A, lda, B, ldb, stream);
+}
+
+rmh_status rmh_set_potrf_blocking(rmh_handle h, size_t nb, size_t nb_inner) {
+    if (h == nullptr) return fail(RMH_STATUS_INVALID_HANDLE, "NULL handle");
+    if (nb == 0) nb = Context::kDefaultNb;
+    if (nb_inner == 0) nb_inner = Context::kDefaultNbi;
+    RMH_REQUIRE(nb_inner <= 64, "set_potrf_blocking: nb_inner must be <= 64");
+    RMH_REQUIRE(nb % nb_inner == 0, "set_potrf_blocking: nb must be a multiple of nb_inner");
+    ctx(h)->nb = nb;
+    ctx(h)->nbi = nb_inner;
+    return RMH_STATUS_SUCCESS;
+}
+
+rmh_status rmh_profile_enable(rmh_handle h, int class_mask) {
+    if (h == nullptr) return fail(RMH_STATUS_INVALID_HANDLE, "NULL handle");
+    ctx(h)->prof_mask = class_mask;
+    return RMH_STATUS_SUCCESS;
+}
+
+rmh_status rmh_profile_reset(rmh_handle h) {
+    if (h == nullptr) return fail(RMH_STATUS_INVALID_HANDLE, "NULL handle");
+    RMH_TRY(ctx(h)->activate());
+    RMH_TRY_HIP(hipDeviceSynchronize());
+    ctx(h)->prof_clear();
+    return RMH_STATUS_SUCCESS;
+}
+
+rmh_status rmh_profile_read(rmh_handle h, const char* kernel_class, double* total_ms, double* total_flops,
+                            int64_t* launches) {
+    if (h == nullptr) return fail(RMH_STATUS_INVALID_HANDLE, "NULL handle");
+    RMH_REQUIRE(kernel_class != nullptr, "profile_read: class is NULL");
+    int cls = -1;
+    for (int i = 0; i < PROF_COUNT; ++i)
+        if (std::strcmp(kernel_class, kProfNames[i]) == 0) cls = i;
+    RMH_REQUIRE(cls >= 0, std::string("profile_read: unknown class ") + kernel_class);
+    RMH_TRY(ctx(h)->activate());
+    double ms = 0.0, fl = 0.0;
+    int64_t cnt = 0;
+    for (auto& r : ctx(h)->recs) {
+        if (r.cls != cls) continue;
+        RMH_TRY_HIP(hipEventSynchronize(r.b));
+        float t = 0.f;
+        RMH_TRY_HIP(hipEventElapsedTime(&t, r.a, r.b));
+        ms += t;
+        fl += r.flops;
+        ++cnt;
+    }
+    if (total_ms) *total_ms = ms;
+    if (total_flops) *total_flops = fl;
+    if (launches) *launches = cnt;
+    return RMH_STATUS_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,325 @@
+// MFMA-tiled GEMM / SYRK for gfx950 (CDNA4).
+//
+// One kernel template serves C = alpha*op(A)*op(B) + beta*C on a full
+// rectangle and the triangular / trapezoidal SYRK updates of the Cholesky
+// factorisation (the trailing update A22 -= P P^T and the in-panel update).
+// It replaces the cuBLAS gemm/syrk calls of the reference's HIP backend
+// (include/remora/hip/cublas_backend.hpp:121-166, 389-431) that
+// include/remora/kernels/hip/{gemm,syrk}.hpp forward to.
+//
+// Design (64-wide wavefronts, 4 waves per workgroup):
+//  * the workgroup tile BM x BN is split 2 x 2 over the waves; each wave
+//    owns (BM/2) x (BN/2) of C as a grid of 16x16 MFMA accumulators
+//    (v_mfma_f64_16x16x4_f64 / v_mfma_f32_16x16x4_f32);
+//  * op(A) and op(B) tiles of depth BK are staged in LDS k-major
+//    ([k][m] and [k][n], rows padded by 16 elements so the 64 lanes of an
+//    MFMA operand fetch -- 16 consecutive m at 4 consecutive k -- hit
+//    distinct banks), double-buffered with a register prefetch of the next
+//    tile so global latency overlaps the MFMAs of the current one;
+//  * the MFMA is issued as D = op(B)^T op(A)^T so that a lane's four
+//    accumulator entries are four columns of C and the 16 lanes of a row
+//    group write 16 consecutive rows: each store instruction writes four
+//    contiguous 16-element column segments of the column-major C;
+//  * workgroup ids are remapped so each XCD (8 per MI355X, with private L2)
+//    receives a contiguous range of tiles, and tiles are visited in groups of
+//    8 tile-rows so neighbouring workgroups share op(A) and op(B) panels;
+//  * Part::Lower / Part::Upper launch only the tiles that touch the wanted
+//    trapezoid and mask the diagonal-crossing ones at the store.
+//
+// In-place use: when C aliases op(A) (or op(B)) and the launch has a single
+// tile column (BN >= N) (or a single tile row), every workgroup reads all of
+// its own rows (columns) before it writes them and no other workgroup reads
+// them.  The Cholesky panel solve uses that (A21 <- A21 * W^T).
+#include "rmh_internal.hpp"
+
+namespace rmh {
+namespace {
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef double acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct Mfma<float> {
+    typedef float acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+template <typename T>
+struct KParams {
+    int64_t M, N, K;
+    T alpha, beta;
+    const T* A;
+    int64_t lda;
+    const T* B;
+    int64_t ldb;
+    T* C;
+    int64_t ldc;
+    int64_t tiles_m, tiles_n;
+    const int64_t* abort_flag;
+};
+
+constexpr int kThreads = 256;
+constexpr int kPad = 16;
+constexpr int kXcd = 8;
+constexpr int kGroupM = 8;
+constexpr int64_t kMaxLd = (int64_t)1 << 23;
+
+// Lower trapezoid of tiles: tile-row i holds min(i+1, tn) tiles.
+__device__ __forceinline__ void lower_trapezoid_tile(int64_t t, int64_t tn, int64_t& ti, int64_t& tj) {
+    const int64_t tri = tn * (tn + 1) / 2;
+    if (t < tri) {
+        int64_t i = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+        while (i * (i + 1) / 2 > t) --i;
+        while ((i + 1) * (i + 2) / 2 <= t) ++i;
+        ti = i;
+        tj = t - i * (i + 1) / 2;
+    } else {
+        const int64_t r = t - tri;
+        ti = tn + r / tn;
+        tj = r % tn;
+    }
+}
+
+template <typename T, int BM, int BN, int BK, bool TA, bool TB, int PART>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void gemm_kernel(KParams<T> p) {
+    typedef Mfma<T> MF;
+    typedef typename MF::acc_t acc_t;
+    constexpr int WM = BM / 2, WN = BN / 2;
+    constexpr int FM = WM / 16, FN = WN / 16;
+    constexpr int LDA_S = BM + kPad, LDB_S = BN + kPad;
+    constexpr int A_PER = BM * BK / kThreads;
+    constexpr int B_PER = BN * BK / kThreads;
+    static_assert(A_PER * kThreads == BM * BK && B_PER * kThreads == BN * BK, "tile/thread mismatch");
+
+    __shared__ T As[2][BK * LDA_S];
+    __shared__ T Bs[2][BK * LDB_S];
+
+    if (p.abort_flag != nullptr && *p.abort_flag != 0) return;
+
+    // ---- tile selection: XCD-contiguous ranges, then grouped order -------
+    const int64_t G = gridDim.x;
+    int64_t pid = blockIdx.x;
+    {
+        const int64_t full = (G / kXcd) * kXcd;
+        if (pid < full) pid = (pid % kXcd) * (G / kXcd) + pid / kXcd;
+    }
+    int64_t tm, tn;
+    if (PART == 0) {
+        const int64_t width = (int64_t)kGroupM * p.tiles_n;
+        const int64_t group = pid / width;
+        const int64_t first = group * kGroupM;
+        const int64_t gsz = min((int64_t)kGroupM, p.tiles_m - first);
+        const int64_t r = pid % width;
+        tm = first + r % gsz;
+        tn = r / gsz;
+    } else if (PART == 1) {
+        lower_trapezoid_tile(pid, p.tiles_n, tm, tn);
+    } else {
+        lower_trapezoid_tile(pid, p.tiles_m, tn, tm);
+    }
+    const int64_t m0 = tm * BM, n0 = tn * BN;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int64_t M = p.M, N = p.N, K = p.K;
+
+    // ---- global -> register staging -------------------------------------
+    // Per-thread coordinates inside a tile are fixed; the tile origin moves
+    // with k0 and is wave-uniform, so loads are a scalar base plus a 32-bit
+    // lane offset.  Out-of-range rows/columns/depths are clamped to a valid
+    // address and zeroed after the load.
+    constexpr int AK_STEP = TA ? 0 : kThreads / BM;   // !TA: k advances by this per r
+    constexpr int AM_STEP = TA ? kThreads / BK : 0;   //  TA: m advances by this per r
+    constexpr int BK_STEP = TB ? kThreads / BN : 0;   //  TB: k advances per r
+    constexpr int BN_STEP = TB ? 0 : kThreads / BK;   // !TB: n advances per r
+    const int a_m0 = TA ? tid / BK : tid % BM;
+    const int a_k0 = TA ? tid % BK : tid / BM;
+    const int b_n0 = TB ? tid % BN : tid / BK;
+    const int b_k0 = TB ? tid / BN : tid % BK;
+    const int lda32 = (int)p.lda, ldb32 = (int)p.ldb;
+
+    T ra[A_PER], rb[B_PER];
+    auto load_tiles = [&](int64_t k0) {
+        const int kmax = (int)min((int64_t)BK, K - k0) - 1;  // >= 0
+        const T* abase = TA ? p.A + k0 + m0 * p.lda : p.A + m0 + k0 * p.lda;
+        const T* bbase = TB ? p.B + n0 + k0 * p.ldb : p.B + k0 + n0 * p.ldb;
+        const int mmax = (int)min((int64_t)BM, M - m0) - 1;
+        const int nmax = (int)min((int64_t)BN, N - n0) - 1;
+#pragma unroll
+        for (int r = 0; r < A_PER; ++r) {
+            const int mm = a_m0 + r * AM_STEP, kk = a_k0 + r * AK_STEP;
+            const bool ok = (mm <= mmax) && (kk <= kmax);
+            const int cm = min(mm, mmax), ck = min(kk, kmax);
+            const T v = TA ? abase[ck + cm * lda32] : abase[cm + ck * lda32];
+            ra[r] = ok ? v : T(0);
+        }
+#pragma unroll
+        for (int r = 0; r < B_PER; ++r) {
+            const int nn = b_n0 + r * BN_STEP, kk = b_k0 + r * BK_STEP;
+            const bool ok = (nn <= nmax) && (kk <= kmax);
+            const int cn = min(nn, nmax), ck = min(kk, kmax);
+            const T v = TB ? bbase[cn + ck * ldb32] : bbase[ck + cn * ldb32];
+            rb[r] = ok ? v : T(0);
+        }
+    };
+    auto store_tiles = [&](int buf) {
+#pragma unroll
+        for (int r = 0; r < A_PER; ++r) {
+            const int mm = a_m0 + r * AM_STEP, kk = a_k0 + r * AK_STEP;
+            As[buf][kk * LDA_S + mm] = ra[r];
+        }
+#pragma unroll
+        for (int r = 0; r < B_PER; ++r) {
+            const int nn = b_n0 + r * BN_STEP, kk = b_k0 + r * BK_STEP;
+            Bs[buf][kk * LDB_S + nn] = rb[r];
+        }
+    };
+
+    acc_t acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+    const int64_t nk = (K + BK - 1) / BK;
+    if (nk > 0) {
+        load_tiles(0);
+        store_tiles(0);
+    }
+    __syncthreads();
+
+    const int lr = lane & 15;   // operand row inside a 16-block
+    const int lk = lane >> 4;   // operand k inside a 4-deep step
+    for (int64_t kt = 0; kt < nk; ++kt) {
+        const int buf = (int)(kt & 1);
+        if (kt + 1 < nk) load_tiles((kt + 1) * BK);
+        const T* as = As[buf];
+        const T* bs = Bs[buf];
+#pragma unroll
+        for (int k4 = 0; k4 < BK / 4; ++k4) {
+            const int kr = k4 * 4 + lk;
+            T a[FM], b[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) a[i] = as[kr * LDA_S + wm * WM + i * 16 + lr];
+#pragma unroll
+            for (int j = 0; j < FN; ++j) b[j] = bs[kr * LDB_S + wn * WN + j * 16 + lr];
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j) acc[i][j] = MF::op(b[j], a[i], acc[i][j]);
+        }
+        if (kt + 1 < nk) store_tiles(buf ^ 1);
+        __syncthreads();
+    }
+
+    // ---- epilogue: lane holds C[m][n..n+3] for 16 consecutive m per group
+    const T alpha = p.alpha, beta = p.beta;
+    T* cbase = p.C + m0 + n0 * p.ldc;
+    const int ldc32 = (int)p.ldc;
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+        const int mm = wm * WM + i * 16 + lr;
+        if (m0 + mm >= M) continue;
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int nn = wn * WN + j * 16 + 4 * lk + r;
+                const int64_t gm = m0 + mm, gn = n0 + nn;
+                bool ok = gn < N;
+                if (PART == 1) ok = ok && (gm >= gn);
+                if (PART == 2) ok = ok && (gm <= gn);
+                if (!ok) continue;
+                T* c = cbase + (mm + nn * ldc32);
+                const T v = alpha * acc[i][j][r];
+                *c = (beta == T(0)) ? v : v + beta * *c;
+            }
+        }
+    }
+}
+
+template <typename T, int BM, int BN, int BK, int PART>
+rmh_status launch_cfg(const GemmDesc<T>& d, hipStream_t stream) {
+    KParams<T> p;
+    p.M = d.M; p.N = d.N; p.K = d.K;
+    p.alpha = d.alpha; p.beta = d.beta;
+    p.A = d.A; p.lda = d.lda; p.B = d.B; p.ldb = d.ldb; p.C = d.C; p.ldc = d.ldc;
+    p.tiles_m = (d.M + BM - 1) / BM;
+    p.tiles_n = (d.N + BN - 1) / BN;
+    p.abort_flag = d.abort_flag;
+    int64_t ntiles;
+    if (PART == 0) {
+        ntiles = p.tiles_m * p.tiles_n;
+    } else if (PART == 1) {  // rows >= cols: tile-row i has min(i+1, tiles_n) tiles
+        const int64_t tn = p.tiles_n, tmm = p.tiles_m;
+        ntiles = tn * (tn + 1) / 2 + (tmm - tn) * tn;
+    } else {
+        const int64_t tn = p.tiles_m, tmm = p.tiles_n;
+        ntiles = tn * (tn + 1) / 2 + (tmm - tn) * tn;
+    }
+    if (ntiles <= 0) return RMH_STATUS_SUCCESS;
+    if (ntiles > 0x7fffffffLL) return fail(RMH_STATUS_INVALID_VALUE, "gemm: problem too large for one launch");
+    dim3 grid((unsigned)ntiles), block(kThreads);
+    if (!d.transA && !d.transB)
+        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, false, false, PART>), grid, block, 0, stream, p);
+    else if (!d.transA && d.transB)
+        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, false, true, PART>), grid, block, 0, stream, p);
+    else if (d.transA && !d.transB)
+        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, true, false, PART>), grid, block, 0, stream, p);
+    else
+        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, true, true, PART>), grid, block, 0, stream, p);
+    return check_launch("gemm_kernel");
+}
+
+template <typename T, int BM, int BN, int BK>
+rmh_status launch_part(const GemmDesc<T>& d, hipStream_t stream) {
+    switch (d.part) {
+        case Part::Full: return launch_cfg<T, BM, BN, BK, 0>(d, stream);
+        case Part::Lower: return launch_cfg<T, BM, BN, BK, 1>(d, stream);
+        case Part::Upper: return launch_cfg<T, BM, BN, BK, 2>(d, stream);
+    }
+    return fail(RMH_STATUS_INVALID_VALUE, "gemm: bad part");
+}
+
+}  // namespace
+
+Tile pick_tile(int64_t M, int64_t N, Part part) {
+    const int64_t t128 = ((M + 127) / 128) * ((N + 127) / 128) / (part == Part::Full ? 1 : 2);
+    if (t128 >= 512) return Tile::T128x128;
+    return Tile::T64x64;
+}
+
+template <typename T>
+rmh_status launch_gemm(const GemmDesc<T>& d, Tile tile, hipStream_t stream) {
+    if (d.M <= 0 || d.N <= 0) return RMH_STATUS_SUCCESS;
+    // tile-relative offsets are 32-bit: (tile extent) * ld must stay below 2^31
+    if (d.lda > kMaxLd || d.ldb > kMaxLd || d.ldc > kMaxLd)
+        return fail(RMH_STATUS_INVALID_VALUE, "gemm: leading dimension above 2^23");
+    if (d.part == Part::Lower && d.M < d.N) return fail(RMH_STATUS_INVALID_VALUE, "gemm: lower part needs M >= N");
+    if (d.part == Part::Upper && d.N < d.M) return fail(RMH_STATUS_INVALID_VALUE, "gemm: upper part needs N >= M");
+    if (d.part != Part::Full && (tile == Tile::T128x64 || tile == Tile::T64x128))
+        return fail(RMH_STATUS_INVALID_VALUE, "gemm: triangular parts need square tiles");
+    constexpr int BK = sizeof(T) == 8 ? 16 : 32;
+    switch (tile) {
+        case Tile::T128x128: return launch_part<T, 128, 128, BK>(d, stream);
+        case Tile::T64x64: return launch_part<T, 64, 64, BK>(d, stream);
+        case Tile::T128x64: return launch_cfg<T, 128, 64, BK, 0>(d, stream);
+        case Tile::T64x128: return launch_cfg<T, 64, 128, BK, 0>(d, stream);
+    }
+    return fail(RMH_STATUS_INVALID_VALUE, "gemm: bad tile");
+}
+
+template rmh_status launch_gemm<double>(const GemmDesc<double>&, Tile, hipStream_t);
+template rmh_status launch_gemm<float>(const GemmDesc<float>&, Tile, hipStream_t);
+
+}  // namespace rmh

@@ -1,0 +1,311 @@
+// Small-block kernels of the Cholesky path: the diagonal-block POTRF, the
+// diagonal-block triangular inverses used by TRSM, the in-place application
+// of an inverted diagonal block, and BLAS-2 helpers.
+//
+// The diagonal-block POTRF replaces potrf_block_kernel of the reference HIP
+// backend (include/remora/kernels/hip/potrf.hpp:41-80), which factored a
+// 16x16 block with a single thread; here one wavefront cooperates on a block
+// of up to 64 x 64 kept in LDS: lane i owns row i, every column step is a
+// broadcast of the pivot column followed by a lane-parallel rank-1 update,
+// and the same wave then inverts the factor (lane c owns column c of L^-1),
+// so the panel solve of the blocked factorisation becomes an MFMA GEMM.
+#include "rmh_internal.hpp"
+
+namespace rmh {
+namespace {
+
+constexpr int kDiagMax = 64;
+constexpr int kLdsLd = kDiagMax + 1;  // row stride of the LDS blocks (odd: conflict-free columns)
+
+// Lower-triangular inverse of the kb x kb factor held in L (LDS, row stride
+// kLdsLd): lane c computes column c of X = L^-1 by forward substitution.
+// No lane reads another lane's column, so no barrier is needed inside.
+__device__ __forceinline__ void invert_lower_lds(const double* L, double* X, int kb, bool unit) {
+    const int c = threadIdx.x;
+    if (c < kb) X[c * kLdsLd + c] = unit ? 1.0 : 1.0 / L[c * kLdsLd + c];
+    for (int i = 1; i < kb; ++i) {
+        double acc = 0.0;
+        for (int t = 0; t < i; ++t) {
+            const double l = L[i * kLdsLd + t];
+            if (t >= c && c < kb) acc = fma(l, X[t * kLdsLd + c], acc);
+        }
+        if (c < i) X[i * kLdsLd + c] = unit ? -acc : -acc / L[i * kLdsLd + i];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void potrf_diag_kernel(T* A, int64_t rs, int64_t cs, int kb, T* W, int64_t ldw,
+                                                        int64_t* info, int64_t info_base) {
+    __shared__ double L[kDiagMax * kLdsLd];
+    __shared__ double X[kDiagMax * kLdsLd];
+    if (*info != 0) return;  // an earlier block already failed: LAPACK stops there
+    const int lane = threadIdx.x;
+
+    // load the lower triangle, lane = row
+    if (lane < kb)
+        for (int j = 0; j <= lane; ++j) L[lane * kLdsLd + j] = (double)A[lane * rs + j * cs];
+    __syncthreads();
+
+    int failed = -1;
+    for (int j = 0; j < kb; ++j) {
+        const double d = L[j * kLdsLd + j];
+        if (!(d > 0.0)) {  // also catches NaN
+            failed = j;
+            break;
+        }
+        const double s = sqrt(d);
+        double lij = 0.0;
+        if (lane > j && lane < kb) {
+            lij = L[lane * kLdsLd + j] / s;
+            L[lane * kLdsLd + j] = lij;
+        }
+        __syncthreads();
+        if (lane == 0) L[j * kLdsLd + j] = s;
+        // rank-1 update of the trailing lower triangle: row `lane`, cols j+1..lane
+        for (int c = j + 1; c < kb; ++c) {
+            const double lcj = L[c * kLdsLd + j];
+            if (lane >= c && lane < kb) L[lane * kLdsLd + c] = fma(-lij, lcj, L[lane * kLdsLd + c]);
+        }
+        __syncthreads();
+    }
+
+    // write back what was computed (all of it on success)
+    if (lane < kb)
+        for (int j = 0; j <= lane; ++j) A[lane * rs + j * cs] = (T)L[lane * kLdsLd + j];
+    if (failed >= 0) {
+        if (lane == 0) *info = info_base + failed + 1;
+        return;
+    }
+    __syncthreads();
+    invert_lower_lds(L, X, kb, false);
+    __syncthreads();
+    // W column-major, zero above the diagonal; lane = row for coalesced stores
+    if (lane < kb)
+        for (int c = 0; c < kb; ++c) W[lane + c * ldw] = (T)(lane >= c ? X[lane * kLdsLd + c] : 0.0);
+}
+
+// Invert the nb x nb diagonal blocks of op(A).  One wavefront per block.
+// An upper block U is handled as the lower block U^T: inv(U) = inv(U^T)^T.
+template <typename T>
+__global__ __launch_bounds__(64) void trtri_blocks_kernel(const T* A, int64_t lda, int64_t n, int nb,
+                                                          bool eff_upper, bool trans, bool unit, T* Dinv) {
+    __shared__ double L[kDiagMax * kLdsLd];
+    __shared__ double X[kDiagMax * kLdsLd];
+    const int64_t b = blockIdx.x;
+    const int64_t r0 = b * nb;
+    const int kb = (int)min((int64_t)nb, n - r0);
+    const int lane = threadIdx.x;
+    // element (i,j) of the effective block op(A)_bb is A[(r0+i) + (r0+j)*lda]
+    // (no transpose) or A[(r0+j) + (r0+i)*lda] (transpose).  Load its lower
+    // form Lf(i,j) = eff(i,j) if lower, eff(j,i) if upper.
+    if (lane < kb) {
+        for (int j = 0; j <= lane; ++j) {
+            int64_t ei = lane, ej = j;       // Lf(i,j) -> eff index
+            if (eff_upper) { ei = j; ej = lane; }
+            int64_t ai = ei, aj = ej;        // eff -> A index
+            if (trans) { ai = ej; aj = ei; }
+            L[lane * kLdsLd + j] = (double)A[(r0 + ai) + (r0 + aj) * lda];
+        }
+    }
+    __syncthreads();
+    invert_lower_lds(L, X, kb, unit);
+    __syncthreads();
+    T* D = Dinv + b * (int64_t)nb * nb;
+    // D(i,j) = inverse of eff block, column-major, full kb x kb (ld = nb)
+    if (lane < kb) {
+        for (int c = 0; c < kb; ++c) {
+            double v;
+            if (!eff_upper) v = lane >= c ? X[lane * kLdsLd + c] : 0.0;  // lower: D = X
+            else v = c >= lane ? X[c * kLdsLd + lane] : 0.0;              // upper: D = X^T
+            D[lane + c * (int64_t)nb] = (T)v;
+        }
+    }
+}
+
+// B_blk <- D * B_blk (left) / B_blk * D (right), kb <= 64, one workgroup per
+// 64-wide slab of the other dimension.
+template <typename T, bool LEFT>
+__global__ __launch_bounds__(256) void tri_apply_kernel(const T* D, int64_t ldd, int kb, T* B, int64_t ldb,
+                                                        int64_t r0, int64_t other) {
+    constexpr int S = 64;
+    __shared__ T Ds[kDiagMax * kLdsLd];
+    __shared__ T Bs[kDiagMax * kLdsLd];
+    const int tid = threadIdx.x;
+    const int64_t o0 = (int64_t)blockIdx.x * S;
+    const int ow = (int)min((int64_t)S, other - o0);
+    for (int e = tid; e < kb * kb; e += 256) {
+        const int i = e % kb, j = e / kb;
+        Ds[i * kLdsLd + j] = D[i + (int64_t)j * ldd];
+    }
+    // slab element (t, o): LEFT: B[(r0+t) + (o0+o)*ldb]  RIGHT: B[(o0+o) + (r0+t)*ldb]
+    for (int e = tid; e < kb * ow; e += 256) {
+        int t, o;
+        if (LEFT) { t = e % kb; o = e / kb; }
+        else { o = e % ow; t = e / ow; }
+        const int64_t idx = LEFT ? (r0 + t) + (o0 + o) * ldb : (o0 + o) + (r0 + t) * ldb;
+        Bs[t * kLdsLd + o] = B[idx];
+    }
+    __syncthreads();
+    for (int e = tid; e < kb * ow; e += 256) {
+        int t, o;
+        if (LEFT) { t = e % kb; o = e / kb; }
+        else { o = e % ow; t = e / ow; }
+        double acc = 0.0;
+        if (LEFT) {  // (D * Bblk)(t, o) = sum_u D(t,u) Bblk(u,o)
+            for (int u = 0; u < kb; ++u) acc = fma((double)Ds[t * kLdsLd + u], (double)Bs[u * kLdsLd + o], acc);
+        } else {     // (Bblk * D)(o, t) = sum_u Bblk(o,u) D(u,t); slab stored as Bs[u][o]
+            for (int u = 0; u < kb; ++u) acc = fma((double)Bs[u * kLdsLd + o], (double)Ds[u * kLdsLd + t], acc);
+        }
+        const int64_t idx = LEFT ? (r0 + t) + (o0 + o) * ldb : (o0 + o) + (r0 + t) * ldb;
+        B[idx] = (T)acc;
+    }
+}
+
+template <typename T>
+__global__ void scale_kernel(T* B, int64_t ldb, int64_t rows, int64_t cols, T alpha) {
+    const int64_t total = rows * cols;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e % rows, j = e / rows;
+        T* b = B + i + j * ldb;
+        *b = alpha == T(0) ? T(0) : alpha * *b;
+    }
+}
+
+// y = alpha*A*x + beta*y: one thread per row, coalesced down the columns.
+template <typename T>
+__global__ void gemv_n_kernel(int64_t m, int64_t n, T alpha, const T* A, int64_t lda, const T* x, int64_t incx,
+                              T beta, T* y, int64_t incy) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    double acc = 0.0;
+    for (int64_t j = 0; j < n; ++j) acc = fma((double)A[i + j * lda], (double)x[j * incx], acc);
+    T* yi = y + i * incy;
+    const T v = alpha * (T)acc;
+    *yi = beta == T(0) ? v : v + beta * *yi;
+}
+
+// y = alpha*A^T*x + beta*y: one workgroup per output, tree reduction.
+template <typename T>
+__global__ __launch_bounds__(256) void gemv_t_kernel(int64_t m, int64_t n, T alpha, const T* A, int64_t lda,
+                                                     const T* x, int64_t incx, T beta, T* y, int64_t incy) {
+    __shared__ double red[256];
+    const int64_t j = blockIdx.x;
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < m; i += 256) acc = fma((double)A[i + j * lda], (double)x[i * incx], acc);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        T* yj = y + j * incy;
+        const T v = alpha * (T)red[0];
+        *yj = beta == T(0) ? v : v + beta * *yj;
+    }
+}
+
+template <typename T>
+__global__ void strided_copy_kernel(const T* src, int64_t sinc, T* dst, int64_t dinc, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) dst[i * dinc] = src[i * sinc];
+}
+
+__global__ void fill_i64_kernel(int64_t* p, int64_t n, int64_t v) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+unsigned grid_for(int64_t n, int block) {
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > 65535LL * 32) g = 65535LL * 32;
+    return (unsigned)g;
+}
+
+}  // namespace
+
+template <typename T>
+rmh_status launch_potrf_diag(T* A, int64_t rs, int64_t cs, int kb, T* W, int64_t ldw, int64_t* info,
+                             int64_t info_base, hipStream_t stream) {
+    if (kb <= 0) return RMH_STATUS_SUCCESS;
+    if (kb > kDiagMax) return fail(RMH_STATUS_INVALID_VALUE, "potrf_diag: block larger than 64");
+    hipLaunchKernelGGL(potrf_diag_kernel<T>, dim3(1), dim3(64), 0, stream, A, rs, cs, kb, W, ldw, info, info_base);
+    return check_launch("potrf_diag_kernel");
+}
+
+template <typename T>
+rmh_status launch_trtri_blocks(const T* A, int64_t lda, int64_t n, int nb, bool upper, bool trans, bool unit,
+                               T* Dinv, hipStream_t stream) {
+    if (n <= 0) return RMH_STATUS_SUCCESS;
+    if (nb > kDiagMax) return fail(RMH_STATUS_INVALID_VALUE, "trtri_blocks: block larger than 64");
+    const bool eff_upper = upper != trans;
+    const int64_t nblk = (n + nb - 1) / nb;
+    hipLaunchKernelGGL(trtri_blocks_kernel<T>, dim3((unsigned)nblk), dim3(64), 0, stream, A, lda, n, nb, eff_upper,
+                       trans, unit, Dinv);
+    return check_launch("trtri_blocks_kernel");
+}
+
+template <typename T>
+rmh_status launch_tri_apply(bool left, const T* Dinv, int64_t ldd, int kb, T* B, int64_t ldb, int64_t r0,
+                            int64_t other, hipStream_t stream) {
+    if (kb <= 0 || other <= 0) return RMH_STATUS_SUCCESS;
+    if (kb > kDiagMax) return fail(RMH_STATUS_INVALID_VALUE, "tri_apply: block larger than 64");
+    const unsigned g = (unsigned)((other + 63) / 64);
+    if (left)
+        hipLaunchKernelGGL((tri_apply_kernel<T, true>), dim3(g), dim3(256), 0, stream, Dinv, ldd, kb, B, ldb, r0, other);
+    else
+        hipLaunchKernelGGL((tri_apply_kernel<T, false>), dim3(g), dim3(256), 0, stream, Dinv, ldd, kb, B, ldb, r0, other);
+    return check_launch("tri_apply_kernel");
+}
+
+template <typename T>
+rmh_status launch_scale(T* B, int64_t ldb, int64_t rows, int64_t cols, T alpha, hipStream_t stream) {
+    if (rows <= 0 || cols <= 0) return RMH_STATUS_SUCCESS;
+    hipLaunchKernelGGL(scale_kernel<T>, dim3(grid_for(rows * cols, 256)), dim3(256), 0, stream, B, ldb, rows, cols, alpha);
+    return check_launch("scale_kernel");
+}
+
+template <typename T>
+rmh_status launch_gemv(bool trans, int64_t m, int64_t n, T alpha, const T* A, int64_t lda, const T* x, int64_t incx,
+                       T beta, T* y, int64_t incy, hipStream_t stream) {
+    if (!trans) {
+        if (m <= 0) return RMH_STATUS_SUCCESS;
+        hipLaunchKernelGGL(gemv_n_kernel<T>, dim3(grid_for(m, 256)), dim3(256), 0, stream, m, n, alpha, A, lda, x, incx,
+                           beta, y, incy);
+    } else {
+        if (n <= 0) return RMH_STATUS_SUCCESS;
+        hipLaunchKernelGGL(gemv_t_kernel<T>, dim3((unsigned)n), dim3(256), 0, stream, m, n, alpha, A, lda, x, incx,
+                           beta, y, incy);
+    }
+    return check_launch("gemv_kernel");
+}
+
+template <typename T>
+rmh_status launch_strided_copy(const T* src, int64_t sinc, T* dst, int64_t dinc, int64_t n, hipStream_t stream) {
+    if (n <= 0) return RMH_STATUS_SUCCESS;
+    hipLaunchKernelGGL(strided_copy_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0, stream, src, sinc, dst, dinc, n);
+    return check_launch("strided_copy_kernel");
+}
+
+rmh_status launch_fill_i64(int64_t* p, int64_t n, int64_t value, hipStream_t stream) {
+    if (n <= 0) return RMH_STATUS_SUCCESS;
+    hipLaunchKernelGGL(fill_i64_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, p, n, value);
+    return check_launch("fill_i64_kernel");
+}
+
+#define RMH_INST(T)                                                                                         \
+    template rmh_status launch_potrf_diag<T>(T*, int64_t, int64_t, int, T*, int64_t, int64_t*, int64_t,     \
+                                             hipStream_t);                                                 \
+    template rmh_status launch_trtri_blocks<T>(const T*, int64_t, int64_t, int, bool, bool, bool, T*,       \
+                                               hipStream_t);                                               \
+    template rmh_status launch_tri_apply<T>(bool, const T*, int64_t, int, T*, int64_t, int64_t, int64_t, hipStream_t); \
+    template rmh_status launch_scale<T>(T*, int64_t, int64_t, int64_t, T, hipStream_t);                    \
+    template rmh_status launch_gemv<T>(bool, int64_t, int64_t, T, const T*, int64_t, const T*, int64_t, T, \
+                                       T*, int64_t, hipStream_t);                                          \
+    template rmh_status launch_strided_copy<T>(const T*, int64_t, T*, int64_t, int64_t, hipStream_t);
+RMH_INST(double)
+RMH_INST(float)
+#undef RMH_INST
+
+}  // namespace rmh

@@ -1,0 +1,45 @@
+// The object behind an rmh_handle: device, scratch memory, tuning and the
+// optional launch profiler.
+#pragma once
+
+#include <vector>
+
+#include "rmh_internal.hpp"
+
+namespace rmh {
+
+enum ProfClass { PROF_TRAILING = 0, PROF_PANEL = 1, PROF_DIAG = 2, PROF_INNER = 3, PROF_COUNT = 4 };
+static const char* const kProfNames[PROF_COUNT] = {"trailing_update", "panel", "diag", "inner_update"};
+
+struct ProfRec {
+    hipEvent_t a = nullptr, b = nullptr;
+    double flops = 0.0;
+    int cls = 0;
+};
+
+struct Context {
+    static constexpr size_t kDefaultNb = 256;
+    static constexpr size_t kDefaultNbi = 64;
+
+    int device = 0;
+    void* ws = nullptr;  // scratch, grown on demand, reused by every call on this handle
+    size_t ws_bytes = 0;
+    size_t nb = kDefaultNb, nbi = kDefaultNbi;
+    int prof_mask = 0;  // bit i: record launches of ProfClass i
+    std::vector<ProfRec> recs;
+
+    rmh_status activate();
+    rmh_status reserve(size_t bytes);
+    rmh_status prof_begin(hipStream_t s, int cls, double flops, int* slot);
+    rmh_status prof_end(hipStream_t s, int slot);
+    void prof_clear();
+    ~Context();
+};
+
+template <typename T>
+rmh_status trsm_impl(Context* h, bool left, bool upper, bool trans, bool unit, int64_t m, int64_t n, T alpha,
+                     const T* A, int64_t lda, T* B, int64_t ldb, hipStream_t s);
+template <typename T>
+rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int64_t* d_info, hipStream_t s);
+
+}  // namespace rmh

@@ -1,0 +1,147 @@
+"""torch-tensor front of the C-ABI (device memory and streams come from torch).
+
+Matrices are column-major: a 2-D CUDA tensor ``X`` of shape (rows, cols)
+with ``X.stride(0) == 1`` and leading dimension ``X.stride(1)``; ``colmajor``
+makes one.  Every call is asynchronous on torch's current stream unless a
+stream is given.  Functions mirror the BLAS-style entry points of
+``include/remora_hip/capi.h`` one to one.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import capi
+
+_DT = {torch.float64: "d", torch.float32: "s"}
+
+
+def colmajor(x: torch.Tensor) -> torch.Tensor:
+    """Column-major copy (or view, if already column-major) of a 2-D tensor."""
+    if x.dim() != 2:
+        raise ValueError("expected a matrix")
+    if x.stride(0) == 1 and x.stride(1) >= max(1, x.size(0)):
+        return x
+    return x.t().contiguous().t()
+
+
+def empty_colmajor(rows: int, cols: int, dtype=torch.float64, device="cuda") -> torch.Tensor:
+    return torch.empty((cols, rows), dtype=dtype, device=device).t()
+
+
+def _ld(x: torch.Tensor) -> int:
+    if x.dim() != 2:
+        raise ValueError("expected a matrix")
+    if x.numel() == 0:
+        return max(1, x.size(0))
+    if x.stride(0) != 1 and x.size(0) > 1:
+        raise ValueError("matrix is not column-major (stride(0) must be 1)")
+    return max(1, x.stride(1) if x.size(1) > 1 else x.size(0))
+
+
+def _p(x: torch.Tensor):
+    return C.c_void_p(x.data_ptr())
+
+
+def _stream(stream):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _fn(prefix: str, dtype, name: str):
+    if dtype not in _DT:
+        raise TypeError(f"unsupported dtype {dtype}")
+    return getattr(capi.load(), f"{prefix}{_DT[dtype]}{name}")
+
+
+def gemm(h, A, B, C_, alpha=1.0, beta=0.0, transA=False, transB=False, stream=None):
+    """C = alpha op(A) op(B) + beta C."""
+    m, n = C_.shape
+    k = A.shape[0] if transA else A.shape[1]
+    f = _fn("rmh_", C_.dtype, "gemm")
+    capi.check(f(h.value, int(transA), int(transB), m, n, k, alpha, _p(A), _ld(A), _p(B), _ld(B), beta, _p(C_),
+                 _ld(C_), _stream(stream)))
+    return C_
+
+
+def syrk(h, A, C_, alpha=1.0, beta=0.0, upper=False, trans=False, stream=None):
+    """C = alpha op(A) op(A)^T + beta C on one triangle."""
+    n = C_.shape[0]
+    k = A.shape[0] if trans else A.shape[1]
+    f = _fn("rmh_", C_.dtype, "syrk")
+    capi.check(f(h.value, int(upper), int(trans), n, k, alpha, _p(A), _ld(A), beta, _p(C_), _ld(C_),
+                 _stream(stream)))
+    return C_
+
+
+def trsm(h, A, B, left=True, upper=False, trans=False, unit=False, alpha=1.0, stream=None):
+    """Solve op(A) X = alpha B (left) or X op(A) = alpha B (right) in place."""
+    m, n = B.shape
+    f = _fn("rmh_", B.dtype, "trsm")
+    capi.check(f(h.value, int(left), int(upper), int(trans), int(unit), m, n, alpha, _p(A), _ld(A), _p(B), _ld(B),
+                 _stream(stream)))
+    return B
+
+
+def trsv(h, A, x, upper=False, trans=False, unit=False, incx=1, stream=None):
+    """Solve op(A) x = b in place; x is a 1-D tensor read with stride incx."""
+    n = A.shape[0]
+    f = _fn("rmh_", A.dtype, "trsv")
+    capi.check(f(h.value, int(upper), int(trans), int(unit), n, _p(A), _ld(A), _p(x), incx, _stream(stream)))
+    return x
+
+
+def gemv(h, A, x, y, alpha=1.0, beta=0.0, trans=False, incx=1, incy=1, stream=None):
+    m, n = A.shape
+    f = _fn("rmh_", A.dtype, "gemv")
+    capi.check(f(h.value, int(trans), m, n, alpha, _p(A), _ld(A), _p(x), incx, beta, _p(y), incy, _stream(stream)))
+    return y
+
+
+def potrf(h, A, upper=False, info=None, stream=None):
+    """In-place Cholesky; returns the device int64 info tensor."""
+    n = A.shape[0]
+    if info is None:
+        info = torch.zeros(1, dtype=torch.int64, device=A.device)
+    f = _fn("rmh_", A.dtype, "potrf")
+    capi.check(f(h.value, int(upper), n, _p(A), _ld(A), _p(info), _stream(stream)))
+    return info
+
+
+def potrs(h, L, B, upper=False, stream=None):
+    n, nrhs = B.shape
+    f = _fn("rmh_", B.dtype, "potrs")
+    capi.check(f(h.value, int(upper), n, nrhs, _p(L), _ld(L), _p(B), _ld(B), _stream(stream)))
+    return B
+
+
+def set_potrf_blocking(h, nb=0, nb_inner=0):
+    capi.check(capi.load().rmh_set_potrf_blocking(h.value, nb, nb_inner))
+
+
+class Profile:
+    """Launch profiler of a handle (HIP events around each launch class)."""
+
+    CLASSES = ("trailing_update", "panel", "diag", "inner_update")
+
+    def __init__(self, h):
+        self.h = h
+        self.lib = capi.load()
+
+    def enable(self, classes=CLASSES):
+        mask = 0
+        for c in classes:
+            mask |= 1 << self.CLASSES.index(c)
+        capi.check(self.lib.rmh_profile_enable(self.h.value, mask))
+
+    def disable(self):
+        capi.check(self.lib.rmh_profile_enable(self.h.value, 0))
+
+    def reset(self):
+        capi.check(self.lib.rmh_profile_reset(self.h.value))
+
+    def read(self, cls: str):
+        ms, fl, n = C.c_double(), C.c_double(), C.c_int64()
+        capi.check(self.lib.rmh_profile_read(self.h.value, cls.encode(), C.byref(ms), C.byref(fl), C.byref(n)))
+        return {"ms": ms.value, "flops": fl.value, "launches": n.value}

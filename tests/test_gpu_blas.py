@@ -1,0 +1,238 @@
+"""GPU parity of the BLAS-3/2 entry points against the CPU oracle.
+
+Tolerances are componentwise, relative to the classical rounding-error scale
+|alpha| |op(A)| |op(B)| + |beta| |C| (a 1e-12 bound for FP64 is the
+north-star "within 1e-12 relative" per element of that scale)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from remora_amd import synth
+from tests import fixtures
+from tests.gpu_util import componentwise_ok, dev, host
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"d": 1e-12, "s": 2e-5}
+
+
+def _op(x, t):
+    return x.T if t else x
+
+
+@pytest.mark.parametrize("ta", [False, True])
+@pytest.mark.parametrize("tb", [False, True])
+@pytest.mark.parametrize("shape", [(80, 50, 33), (1, 1, 1), (300, 257, 129), (1031, 777, 513)])
+def test_dgemm_vs_oracle(handle, ta, tb, shape):
+    import torch
+    from remora_amd import linalg as la
+    m, n, k = shape
+    rng_seed = m * 7 + n * 3 + k
+    A = synth.normal(rng_seed, 0, m * k).reshape((k, m) if ta else (m, k))
+    B = synth.normal(rng_seed, 1, k * n).reshape((n, k) if tb else (k, n))
+    C0 = np.full((m, n), 1.5)
+    ref = orc.gemm(_op(A, ta), _op(B, tb), C0, -2.0)  # C0 + -2 op(A) op(B)
+    dC = dev(C0, ld=m + 5)
+    la.gemm(handle, dev(A), dev(B), dC, alpha=-2.0, beta=1.0, transA=ta, transB=tb)
+    got = host(dC)
+    scale = 2.0 * np.abs(_op(A, ta)) @ np.abs(_op(B, tb)) + 1.5
+    ok, worst = componentwise_ok(got, ref, scale, TOL["d"])
+    assert ok, f"worst err/bound {worst}"
+
+
+def test_dgemm_reference_prod_fixture(handle):
+    """Test/prod.cpp:353-380: result(1.5) += -2 * arg1 * arg2, textbook check
+    at 1e-10 percent."""
+    from remora_amd import linalg as la
+    A, B = fixtures.prod_operands()
+    dC = dev(np.full((80, 50), 1.5))
+    la.gemm(handle, dev(A), dev(B), dC, alpha=-2.0, beta=1.0)
+    got = host(dC)
+    ref = 1.5 - 2.0 * (A @ B)
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=0)
+
+
+def test_dgemm_beta_zero_ignores_nan(handle):
+    from remora_amd import linalg as la
+    A = synth.normal(5, 0, 64 * 64).reshape(64, 64)
+    dC = dev(np.full((64, 64), np.nan))
+    la.gemm(handle, dev(A), dev(A), dC, alpha=1.0, beta=0.0)
+    got = host(dC)
+    assert np.all(np.isfinite(got))
+    np.testing.assert_allclose(got, A @ A, rtol=1e-12, atol=1e-12 * np.abs(A) @ np.abs(A))
+
+
+@pytest.mark.parametrize("ta", [False, True])
+@pytest.mark.parametrize("tb", [False, True])
+def test_sgemm(handle, ta, tb):
+    import torch
+    from remora_amd import linalg as la
+    m, n, k = 517, 389, 1039
+    A = synth.normal(11, 0, m * k).reshape((k, m) if ta else (m, k))
+    B = synth.normal(11, 1, k * n).reshape((n, k) if tb else (k, n))
+    dC = dev(np.zeros((m, n)), dtype=torch.float32)
+    la.gemm(handle, dev(A, torch.float32), dev(B, torch.float32), dC, transA=ta, transB=tb)
+    Af = _op(A, ta).astype(np.float32).astype(np.float64)
+    Bf = _op(B, tb).astype(np.float32).astype(np.float64)
+    ok, worst = componentwise_ok(host(dC), Af @ Bf, np.abs(Af) @ np.abs(Bf), TOL["s"])
+    assert ok, worst
+
+
+@pytest.mark.parametrize("upper", [False, True])
+@pytest.mark.parametrize("e_row_major", [True, False])
+def test_dsyrk_reference_fixture(handle, upper, e_row_major):
+    """Test/syrk.cpp:48-87: dims 384, K 244, alpha 2, init 3; the touched
+    triangle is alpha*<row_i,row_j> + init, the other stays init."""
+    from remora_amd import linalg as la
+    arg = fixtures.syrk_arg()
+    dims, K = arg.shape
+    init = np.full((dims, dims), 3.0)
+    # a row-major E is the column-major E^T: the trans=True form of the ABI
+    if e_row_major:
+        dE = dev(arg.T.copy())
+        trans = True
+    else:
+        dE = dev(arg)
+        trans = False
+    dC = dev(init)
+    la.syrk(handle, dE, dC, alpha=2.0, beta=1.0, upper=upper, trans=trans)
+    got = host(dC)
+    full = 2.0 * arg @ arg.T + 3.0
+    mask = np.triu(np.ones((dims, dims), bool)) if upper else np.tril(np.ones((dims, dims), bool))
+    np.testing.assert_allclose(got[mask], full[mask], rtol=1e-12)
+    assert np.all(got[~mask] == 3.0)
+    ref = orc.syrk(arg, init, 2.0, upper)
+    np.testing.assert_allclose(got[mask], ref[mask], rtol=1e-12)
+
+
+@pytest.mark.parametrize("upper", [False, True])
+@pytest.mark.parametrize("trans", [False, True])
+@pytest.mark.parametrize("nk", [(936, 1039), (1, 7), (4099, 256)])
+def test_dsyrk_random(handle, upper, trans, nk):
+    from remora_amd import linalg as la
+    n, k = nk
+    E = synth.normal(n + k, 0, n * k).reshape(n, k)
+    C0 = synth.normal(n + k, 1, n * n).reshape(n, n)
+    dC = dev(C0, ld=n + 3)
+    la.syrk(handle, dev(E.T.copy() if trans else E), dC, alpha=-1.0, beta=0.5, upper=upper, trans=trans)
+    got = host(dC)
+    ref = 0.5 * C0 - E @ E.T
+    mask = np.triu(np.ones((n, n), bool)) if upper else np.tril(np.ones((n, n), bool))
+    scale = np.abs(E) @ np.abs(E.T) + 0.5 * np.abs(C0)
+    ok, worst = componentwise_ok(got[mask], ref[mask], scale[mask], TOL["d"])
+    assert ok, worst
+    assert np.array_equal(got[~mask], C0[~mask])
+
+
+def test_ssyrk_reference_hip_fixture(handle):
+    """Test/hip_syrk.cpp:17-98 (float, dims 936, K 1039): closed form in
+    double on the host, 1e-4 percent."""
+    import torch
+    from remora_amd import linalg as la
+    arg = fixtures.syrk_arg(936, 1039)
+    dC = dev(np.full((936, 936), 3.0), dtype=torch.float32)
+    la.syrk(handle, dev(arg, torch.float32), dC, alpha=2.0, beta=1.0, upper=False)
+    got = host(dC)
+    a32 = arg.astype(np.float32).astype(np.float64)
+    ref = 2.0 * a32 @ a32.T + 3.0
+    mask = np.tril(np.ones((936, 936), bool))
+    np.testing.assert_allclose(got[mask], ref[mask], rtol=1e-6)
+
+
+CASES = [(left, upper, trans, unit) for left in (True, False) for upper in (False, True)
+         for trans in (False, True) for unit in (False, True)]
+
+
+@pytest.mark.parametrize("left,upper,trans,unit", CASES)
+def test_dtrsm_reference_fixture(handle, left, upper, trans, unit):
+    """Test/triangular_solve.cpp:117-212 matrices (158 x 138), every side,
+    triangle, transpose and unit-diagonal combination; checked against the
+    oracle's trsm and by the residual op(A) X = B."""
+    from remora_amd import linalg as la
+    Alow = fixtures.triangular_matrix(158, True)
+    A = Alow if not upper else Alow.T.copy()       # the other triangle holds filler ones
+    B = fixtures.triangular_rhs(158, 138)
+    if not left:
+        B = B.T.copy()                              # 138 x 158
+    dB = dev(B, ld=B.shape[0] + 2)
+    la.trsm(handle, dev(A), dB, left=left, upper=upper, trans=trans, unit=unit)
+    X = host(dB)
+    T = np.tril(A) if not upper else np.triu(A)
+    if unit:
+        T = T - np.diag(np.diag(T)) + np.eye(158)
+    opT = T.T if trans else T
+    # oracle: op(A) triangle as a Remora triangular solve on opT
+    eff_upper = upper != trans
+    ref, sing = orc.trsm(opT, B, upper=eff_upper, unit=False, left=left)
+    assert not sing
+    np.testing.assert_allclose(X, ref, rtol=1e-12, atol=1e-14 * np.max(np.abs(ref)))
+    resid = (opT @ X - B) if left else (X @ opT - B)
+    assert np.max(np.abs(resid)) < 1e-11
+
+
+@pytest.mark.parametrize("left,upper,trans,unit", CASES[::3])
+def test_dtrsm_random_large(handle, left, upper, trans, unit):
+    from remora_amd import linalg as la
+    n, other = 1000, 77
+    L = synth.uniform(3, 0, n * n, -1, 1).reshape(n, n) / n + np.eye(n) * 2.0
+    A = np.tril(L) if not upper else np.triu(L)
+    B = synth.uniform(3, 1, n * other, -1, 1).reshape((n, other) if left else (other, n))
+    dB = dev(B)
+    la.trsm(handle, dev(A), dB, left=left, upper=upper, trans=trans, unit=unit, alpha=0.5)
+    X = host(dB)
+    T = A.copy()
+    if unit:
+        T = T - np.diag(np.diag(T)) + np.eye(n)
+    opT = T.T if trans else T
+    resid = (opT @ X - 0.5 * B) if left else (X @ opT - 0.5 * B)
+    scale = (np.abs(opT) @ np.abs(X)) if left else (np.abs(X) @ np.abs(opT))
+    assert np.max(np.abs(resid) / scale) < 1e-13
+
+
+@pytest.mark.parametrize("upper", [False, True])
+@pytest.mark.parametrize("trans", [False, True])
+@pytest.mark.parametrize("incx", [1, 3])
+def test_dtrsv(handle, upper, trans, incx):
+    """Test/triangular_solve.cpp:19-60 vector case (158)."""
+    import torch
+    from remora_amd import linalg as la
+    Alow = fixtures.triangular_matrix(158, False)
+    A = Alow if not upper else Alow.T.copy()
+    b = (0.1 / 158) * np.arange(158)
+    xbuf = torch.zeros(158 * incx, dtype=torch.float64)
+    xbuf[::incx] = torch.from_numpy(b)
+    dx = xbuf.cuda()
+    la.trsv(handle, dev(A), dx, upper=upper, trans=trans, incx=incx)
+    x = host(dx.view(-1, 1))[::incx, 0]
+    T = np.tril(A) if not upper else np.triu(A)
+    opT = T.T if trans else T
+    assert np.max(np.abs(opT @ x - b)) < 1e-11
+
+
+@pytest.mark.parametrize("trans", [False, True])
+def test_dgemv(handle, trans):
+    import torch
+    from remora_amd import linalg as la
+    m, n = 700, 333
+    A = synth.normal(9, 0, m * n).reshape(m, n)
+    x = synth.normal(9, 1, m if trans else n)
+    y0 = synth.normal(9, 2, n if trans else m)
+    dy = torch.from_numpy(y0.copy()).cuda()
+    la.gemv(handle, dev(A), torch.from_numpy(x).cuda(), dy, alpha=2.0, beta=-1.0, trans=trans)
+    torch.cuda.synchronize()
+    got = dy.cpu().numpy()
+    opA = A.T if trans else A
+    ref = 2.0 * opA @ x - y0
+    scale = 2.0 * np.abs(opA) @ np.abs(x) + np.abs(y0)
+    assert np.all(np.abs(got - ref) <= 1e-13 * scale)
+
+
+def test_invalid_arguments(handle):
+    import ctypes as C
+    from remora_amd import capi
+    lib = capi.load()
+    st = lib.rmh_dgemm(handle.value, 0, 0, 10, 10, 10, 1.0, None, 5, None, 10, 0.0, None, 10, None)
+    assert st == capi.INVALID_VALUE
+    assert b"lda" in lib.rmh_last_error()
+    st = lib.rmh_dpotrf(None, 0, 4, None, 4, None, None)
+    assert st == capi.INVALID_HANDLE

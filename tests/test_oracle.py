@@ -1,0 +1,282 @@
+"""Pins of the CPU oracle (oracle/remora_cpu.c), no GPU needed.
+
+The reference ships no golden vectors for this path; its tests are closed-form
+generators plus invariants.  The oracle is pinned by
+  * those reference tests, restated (fixtures cited per test);
+  * LAPACK (scipy) and 50-digit mpmath Cholesky / triangular solves;
+  * special cases (identity, diagonal, failing pivots);
+  * planted mistakes: each mutant build of the oracle must fail a pin.
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from oracle import build as orc_build
+from oracle import oracle as orc
+from remora_amd import synth
+from tests import fixtures
+
+ORIENT = [True, False]
+
+
+def _mp_cholesky(A, dps=50):
+    import mpmath
+    mpmath.mp.dps = dps
+    n = A.shape[0]
+    M = mpmath.matrix(A.tolist())
+    L = mpmath.matrix(n, n)
+    for j in range(n):
+        s = M[j, j] - sum(L[j, k] ** 2 for k in range(j))
+        L[j, j] = mpmath.sqrt(s)
+        for i in range(j + 1, n):
+            L[i, j] = (M[i, j] - sum(L[i, k] * L[j, k] for k in range(j))) / L[j, j]
+    return np.array([[float(L[i, j]) for j in range(n)] for i in range(n)])
+
+
+# ---------------------------------------------------------------- potrf --
+@pytest.mark.parametrize("row_major", ORIENT)
+def test_potrf_reference_fixture(row_major):
+    """Test/cholesky.cpp:40-80 on createSymm(123)."""
+    A = fixtures.create_symm(123)
+    low, i1 = orc.potrf(A, upper=False, row_major=row_major)
+    up, i2 = orc.potrf(A, upper=True, row_major=row_major)
+    assert i1 == 0 and i2 == 0
+    iu = np.triu_indices(123, 1)
+    np.testing.assert_allclose(low[iu], A[iu], rtol=1e-14)
+    np.testing.assert_allclose(up.T[iu], A.T[iu], rtol=1e-14)
+    L, U = np.tril(low), np.triu(up)
+    assert np.max(np.abs(A - L @ L.T)) < 1e-12
+    assert np.max(np.abs(A - U.T @ U)) < 1e-12
+    assert np.all(np.isfinite(L)) and np.all(np.isfinite(U))
+    # cholesky_decomposition yields the same factor (decompositions.hpp:81-84)
+    _, Ldec, info, _ = orc.posv(A, np.zeros((123, 1)), a_row_major=row_major)
+    assert info == 0
+    np.testing.assert_allclose(np.tril(Ldec), L, rtol=1e-14)
+    np.testing.assert_allclose(U, L.T, rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("n", [1, 2, 31, 32, 33, 64, 123, 200, 517])
+@pytest.mark.parametrize("row_major", ORIENT)
+def test_potrf_vs_lapack(n, row_major):
+    A = synth.gram_spd(n, seed=n, divisor=max(n, 1))
+    F, info = orc.potrf(A, row_major=row_major)
+    assert info == 0
+    L = np.tril(F)
+    Llap = sla.cholesky(A, lower=True)
+    scale = np.sqrt(np.outer(np.diag(A), np.diag(A)))
+    assert np.max(np.abs(L - Llap) / scale) < 50 * n * np.finfo(float).eps
+
+
+@pytest.mark.parametrize("n", [5, 33, 40])
+def test_potrf_vs_mpmath(n):
+    """Exact (50-digit) Cholesky: the oracle is within c*n*u of it."""
+    A = fixtures.create_symm(n)
+    F, info = orc.potrf(A)
+    assert info == 0
+    Lx = _mp_cholesky(A)
+    scale = np.sqrt(np.outer(np.diag(A), np.diag(A)))
+    assert np.max(np.abs(np.tril(F) - Lx) / scale) < 4 * n * np.finfo(float).eps
+
+
+def test_potrf_special_cases():
+    I = np.eye(70)
+    F, info = orc.potrf(I)
+    assert info == 0 and np.array_equal(np.tril(F), I)
+    d = np.arange(1.0, 71.0)
+    F, info = orc.potrf(np.diag(d), upper=True)
+    assert info == 0
+    np.testing.assert_allclose(np.diag(F), np.sqrt(d), rtol=0, atol=0)
+    _, info = orc.potrf(-np.eye(10))
+    assert info == 1
+
+
+def test_potrf_info_is_leaf_relative():
+    """The reference's info is the failing row inside its 32-wide leaf
+    (default/potrf.hpp:55-56 on the leaf view; the recursion returns it
+    unchanged).  A failure at global row 40 (leaf 32..63) reports 9."""
+    n = 100
+    A = synth.gram_spd(n, seed=9)
+    L = np.linalg.cholesky(A[:41, :41])
+    A[40, 40] -= 1.5 * L[40, 40] ** 2
+    _, info = orc.potrf(A)
+    assert info == 40 - 32 + 1
+    _, info = orc.potrf(A, row_major=False)
+    assert info != 0
+
+
+# ----------------------------------------------------------------- trsm --
+TRSM_CASES = [(left, upper, unit) for left in (True, False) for upper in (False, True) for unit in (False, True)]
+
+
+@pytest.mark.parametrize("left,upper,unit", TRSM_CASES)
+@pytest.mark.parametrize("b_row_major", ORIENT)
+def test_trsm_reference_fixture(left, upper, unit, b_row_major):
+    """Test/triangular_solve.cpp:117-212: residual < 1e-11 and agreement
+    with an independent triangular solver (LAPACK trtrs via scipy)."""
+    Alow = fixtures.triangular_matrix(158, True)
+    A = Alow if not upper else Alow.T.copy()
+    B = fixtures.triangular_rhs(158, 138)
+    if not left:
+        B = B.T.copy()
+    X, sing = orc.trsm(A, B, upper, unit, left, b_row_major=b_row_major)
+    assert not sing
+    T = np.triu(A) if upper else np.tril(A)
+    if unit:
+        T = T - np.diag(np.diag(T)) + np.eye(158)
+    R = T @ X - B if left else X @ T - B
+    assert np.max(np.abs(R)) < 1e-11
+    if left:
+        ref = sla.solve_triangular(T, B, lower=not upper)
+    else:
+        ref = sla.solve_triangular(T.T, B.T, lower=upper).T
+    np.testing.assert_allclose(X, ref, rtol=1e-12, atol=1e-14 * np.max(np.abs(ref)))
+
+
+def test_trsm_vs_mpmath_small():
+    import mpmath
+    mpmath.mp.dps = 40
+    n = 37
+    A = np.tril(synth.uniform(1, 0, n * n, -1, 1).reshape(n, n)) + 3 * np.eye(n)
+    b = synth.uniform(1, 1, n, -1, 1).reshape(n, 1)
+    X, _ = orc.trsm(A, b, False, False, True)
+    M = mpmath.matrix(A.tolist())
+    x = mpmath.lu_solve(M, mpmath.matrix(b.tolist()))
+    xe = np.array([float(v) for v in x]).reshape(n, 1)
+    assert np.max(np.abs(X - xe)) < 1e-14 * np.max(np.abs(xe))
+
+
+def test_trsm_singular_is_reported():
+    """A zero diagonal makes the reference throw std::invalid_argument
+    (default/trsm.hpp:86-87)."""
+    A = np.tril(np.ones((50, 50)))
+    A[20, 20] = 0.0
+    _, sing = orc.trsm(A, np.ones((50, 3)), False, False, True)
+    assert sing
+    _, sing = orc.trsm(A, np.ones((50, 3)), False, True, True)
+    assert not sing
+
+
+# ----------------------------------------------------------------- syrk --
+@pytest.mark.parametrize("upper", [False, True])
+@pytest.mark.parametrize("e_rm", ORIENT)
+@pytest.mark.parametrize("m_rm", ORIENT)
+def test_syrk_reference_fixture(upper, e_rm, m_rm):
+    """Test/syrk.cpp:13-87: closed form alpha<row_i,row_j> + init on the
+    touched triangle at 1e-10 percent, the other triangle untouched."""
+    arg = fixtures.syrk_arg()
+    dims = arg.shape[0]
+    M = orc.syrk(arg, np.full((dims, dims), 3.0), 2.0, upper, e_row_major=e_rm, m_row_major=m_rm)
+    full = 2.0 * arg @ arg.T + 3.0
+    mask = np.triu(np.ones((dims, dims), bool)) if upper else np.tril(np.ones((dims, dims), bool))
+    np.testing.assert_allclose(M[mask], full[mask], rtol=1e-12)
+    assert np.all(M[~mask] == 3.0)
+
+
+# ----------------------------------------------------------------- gemm --
+@pytest.mark.parametrize("e1_rm", ORIENT)
+@pytest.mark.parametrize("e2_rm", ORIENT)
+@pytest.mark.parametrize("m_rm", ORIENT)
+def test_gemm_reference_fixture(e1_rm, e2_rm, m_rm):
+    """Test/prod.cpp:330-472: result(1.5) += -2 * arg1 * arg2 for all eight
+    orientation combinations against the textbook triple loop."""
+    A, B = fixtures.prod_operands()
+    got = orc.gemm(A, B, np.full((80, 50), 1.5), -2.0, e1_rm, e2_rm, m_rm)
+    ref = np.full((80, 50), 1.5)
+    for i in range(80):
+        for j in range(50):
+            s = 1.5
+            for k in range(33):
+                s += -2.0 * A[i, k] * B[k, j]
+            ref[i, j] = s
+    np.testing.assert_allclose(got, ref, rtol=1e-12)
+
+
+@pytest.mark.parametrize("mnk", [(1, 1, 1), (257, 1031, 600), (129, 13, 1100)])
+def test_gemm_blocking_edges(mnk):
+    """Sizes that cross the mc=128 / kc=512 / nc=1020 blocks."""
+    m, n, k = mnk
+    A = synth.normal(2, 0, m * k).reshape(m, k)
+    B = synth.normal(2, 1, k * n).reshape(k, n)
+    got = orc.gemm(A, B, np.zeros((m, n)), 1.0)
+    scale = np.abs(A) @ np.abs(B)
+    assert np.all(np.abs(got - A @ B) <= 4 * k * np.finfo(float).eps * scale)
+
+
+# ---------------------------------------------------------------- solve --
+@pytest.mark.parametrize("left", [True, False])
+@pytest.mark.parametrize("b_rm", ORIENT)
+def test_symm_solve_reference_fixture(left, b_rm):
+    """Test/symm_solve.cpp:89-139 (n=128, k=151): max|A X - B| < 1e-12."""
+    A = fixtures.create_symm(128)
+    B = fixtures.symm_solve_rhs(128, 151)
+    if not left:
+        B = B.T.copy()
+    X, _, info, sing = orc.posv(A, B, left=left, b_row_major=b_rm)
+    assert info == 0 and not sing
+    R = A @ X - B if left else X @ A - B
+    assert np.max(np.abs(R)) < 1e-12
+    ref = np.linalg.solve(A, B) if left else np.linalg.solve(A.T, B.T).T
+    np.testing.assert_allclose(X, ref, rtol=1e-12, atol=1e-14)
+
+
+def test_cpu_config_backward_error():
+    """The 512-point CPU configuration: A = G G^T/512 + 0.1 I (seed 0),
+    16 RHS U[-1,1]."""
+    A = synth.gram_spd(512, seed=0, divisor=512)
+    B = synth.rhs(512, 16, seed=0)
+    X, L, info, _ = orc.posv(A, B, a_row_major=True, b_row_major=True)
+    assert info == 0
+    assert synth.factor_error(A, L) < 1e-15
+    assert synth.backward_error(A, X, B) < 1e-16
+
+
+def test_block_constants():
+    """AVX build: 32-byte vectors (simd.hpp:49-53) with REMORA_USE_SIMD."""
+    assert orc.vector_length() == 32
+    assert orc.simd_enabled()
+
+
+# -------------------------------------------------------------- mutants --
+@pytest.fixture(scope="module")
+def mutant_dir(tmp_path_factory):
+    return tmp_path_factory.mktemp("mutants")
+
+
+def _mutant(mutant_dir, name):
+    path = os.path.join(str(mutant_dir), f"liboracle_{name}.so")
+    if not os.path.exists(path):
+        orc_build.build(path, mutant=name)
+    return orc.load(path)
+
+
+def test_mutant_ugemm_dropped_term_is_caught(mutant_dir):
+    lib = _mutant(mutant_dir, "UGEMM_DROP_LAST_K")
+    A, B = fixtures.prod_operands()
+    got = orc.gemm(A, B, np.full((80, 50), 1.5), -2.0, lib=lib)
+    assert not np.allclose(got, 1.5 - 2.0 * A @ B, rtol=1e-12)
+
+
+def test_mutant_trsm_transposed_is_caught(mutant_dir):
+    lib = _mutant(mutant_dir, "TRSM_TRANSPOSED_A")
+    A = fixtures.triangular_matrix(158, True)
+    B = fixtures.triangular_rhs(158, 138)
+    X, _ = orc.trsm(A, B, False, False, True, lib=lib)
+    assert np.max(np.abs(np.tril(A) @ X - B)) > 1e-11
+
+
+def test_mutant_potrf_sign_is_caught(mutant_dir):
+    lib = _mutant(mutant_dir, "POTRF_SIGN")
+    A = fixtures.create_symm(123)
+    F, info = orc.potrf(A, lib=lib)
+    L = np.tril(F)
+    assert info != 0 or not np.all(np.isfinite(L)) or np.max(np.abs(A - L @ L.T)) > 1e-12
+
+
+def test_mutant_syrk_triangle_is_caught(mutant_dir):
+    lib = _mutant(mutant_dir, "SYRK_WRONG_TRIANGLE")
+    arg = fixtures.syrk_arg()
+    M = orc.syrk(arg, np.full((384, 384), 3.0), 2.0, False, lib=lib)
+    mask = np.tril(np.ones((384, 384), bool))
+    assert not (np.allclose(M[mask], (2.0 * arg @ arg.T + 3.0)[mask], rtol=1e-12) and np.all(M[~mask] == 3.0))
