@@ -1,0 +1,265 @@
+// remora/hip/dense.hpp -- dense matrices and vectors of the hip_tag
+// front-end (and the minimal cpu_tag matrix the transport needs).
+//
+// Mirrors (same names and meaning, own code):
+//   matrix_expression / matrix_container / vector_expression CRTP bases ... include/remora/expression_types.hpp:45-115
+//   dense_matrix_storage{values, leading_dimension} ....................... include/remora/detail/storage.hpp:71-91
+//   matrix<T, Orientation, hip_tag> ........................................ include/remora/hip/dense.hpp:451-656
+//   dense_matrix_adaptor (views), trans(), subrange() ...................... include/remora/proxy_expressions.hpp:88-99, 210-240
+//   copy_to_device / copy_to_cpu ........................................... include/remora/hip/copy.hpp:468-502
+// Storage: row_major keeps element (i,j) at values[i*ld + j], column_major
+// at values[i + j*ld] (detail/storage.hpp, detail/structure.hpp:96,141).
+#ifndef REMORA_HIP_DENSE_HPP
+#define REMORA_HIP_DENSE_HPP
+
+#include <algorithm>
+#include <cassert>
+#include <type_traits>
+#include <vector>
+
+#include "core.hpp"
+
+namespace remora {
+
+template <class E, class Device>
+struct matrix_expression {
+    typedef Device device_type;
+    E& operator()() { return *static_cast<E*>(this); }
+    E const& operator()() const { return *static_cast<E const*>(this); }
+};
+template <class C, class Device>
+struct matrix_container : public matrix_expression<C, Device> {
+    C& operator()() { return *static_cast<C*>(this); }
+    C const& operator()() const { return *static_cast<C const*>(this); }
+};
+template <class E, class Device>
+struct vector_expression {
+    typedef Device device_type;
+    E& operator()() { return *static_cast<E*>(this); }
+    E const& operator()() const { return *static_cast<E const*>(this); }
+};
+
+template <class T>
+struct dense_matrix_storage {
+    T* values;
+    std::size_t leading_dimension;
+};
+template <class T>
+struct dense_vector_storage {
+    T* values;
+    std::size_t stride;
+};
+
+namespace detail {
+template <class O>
+inline std::size_t element_index(std::size_t i, std::size_t j, std::size_t ld) {
+    return O::is_row_major ? i * ld + j : i + j * ld;
+}
+}  // namespace detail
+
+// ---------------------------------------------------------------- views --
+// Non-owning view of a dense device matrix (the role of
+// dense_matrix_adaptor<T, Orientation, hip_tag>).
+template <class T, class Orientation, class Device>
+class matrix_view;
+
+template <class T, class Orientation>
+class matrix_view<T, Orientation, hip_tag> : public matrix_expression<matrix_view<T, Orientation, hip_tag>, hip_tag> {
+public:
+    typedef T value_type;
+    typedef Orientation orientation;
+    typedef dense_matrix_storage<T> storage_type;
+    matrix_view(T* values, std::size_t size1, std::size_t size2, std::size_t ld, hip::device& q)
+        : m_values(values), m_size1(size1), m_size2(size2), m_ld(ld), m_queue(&q) {}
+    std::size_t size1() const { return m_size1; }
+    std::size_t size2() const { return m_size2; }
+    storage_type raw_storage() const { return {m_values, m_ld}; }
+    hip::device& queue() const { return *m_queue; }
+
+private:
+    T* m_values;
+    std::size_t m_size1, m_size2, m_ld;
+    hip::device* m_queue;
+};
+
+// ---------------------------------------------------- device container --
+template <class T, class Orientation = row_major, class Device = cpu_tag>
+class matrix;
+
+template <class T, class Orientation>
+class matrix<T, Orientation, hip_tag> : public matrix_container<matrix<T, Orientation, hip_tag>, hip_tag> {
+public:
+    typedef T value_type;
+    typedef Orientation orientation;
+    typedef dense_matrix_storage<T> storage_type;
+    typedef hip_tag device_type;
+
+    explicit matrix(hip::device& q = hip::device_manager::instance().default_device()) : m_queue(&q) {}
+    matrix(std::size_t size1, std::size_t size2,
+           hip::device& q = hip::device_manager::instance().default_device())
+        : m_queue(&q), m_size1(size1), m_size2(size2), m_data(size1 * size2, q) {}
+    matrix(matrix const& o) : matrix(o.size1(), o.size2(), o.queue()) { assign_from(o); }
+    matrix(matrix&&) noexcept = default;
+    template <class E>
+    matrix(matrix_expression<E, hip_tag> const& e) : matrix(e().size1(), e().size2(), e().queue()) {
+        assign_from(e());
+    }
+    matrix& operator=(matrix const& o) {
+        resize(o.size1(), o.size2());
+        assign_from(o);
+        return *this;
+    }
+    matrix& operator=(matrix&&) noexcept = default;
+    template <class E>
+    matrix& operator=(matrix_expression<E, hip_tag> const& e) {
+        resize(e().size1(), e().size2());
+        assign_from(e());
+        return *this;
+    }
+
+    std::size_t size1() const { return m_size1; }
+    std::size_t size2() const { return m_size2; }
+    std::size_t leading_dimension() const { return std::max<std::size_t>(1, Orientation::is_row_major ? m_size2 : m_size1); }
+    storage_type raw_storage() const { return {m_data.get(), leading_dimension()}; }
+    hip::device& queue() const { return *m_queue; }
+    void resize(std::size_t s1, std::size_t s2) {
+        if (s1 * s2 != m_data.size()) m_data = hip::buffer<T>(s1 * s2, *m_queue);
+        m_size1 = s1;
+        m_size2 = s2;
+    }
+
+private:
+    // element-wise copy of any dense device view, orientation-aware
+    template <class E>
+    void assign_from(E const& e) {
+        auto src = e.raw_storage();
+        auto dst = raw_storage();
+        hip::stream& s = hip::get_stream(*m_queue);
+        m_queue->set_device();
+        if (m_size1 == 0 || m_size2 == 0) return;
+        if (std::is_same<typename E::orientation, Orientation>::value) {
+            // same layout: 2-D copy of "lines" (rows for row_major, columns otherwise)
+            const std::size_t lines = Orientation::is_row_major ? m_size1 : m_size2;
+            const std::size_t len = Orientation::is_row_major ? m_size2 : m_size1;
+            hip::check_hip(hipMemcpy2DAsync(dst.values, dst.leading_dimension * sizeof(T), src.values,
+                                            src.leading_dimension * sizeof(T), len * sizeof(T), lines,
+                                            hipMemcpyDeviceToDevice, s.handle()),
+                           "hipMemcpy2DAsync");
+        } else {
+            // transposed layouts: bounce through the host (tests / setup only)
+            std::vector<T> h(src.leading_dimension * (E::orientation::is_row_major ? e.size1() : e.size2()));
+            hip::check_hip(hipMemcpyAsync(h.data(), src.values, h.size() * sizeof(T), hipMemcpyDeviceToHost,
+                                          s.handle()), "hipMemcpyAsync");
+            s.synchronize();
+            std::vector<T> out(m_size1 * m_size2);
+            for (std::size_t i = 0; i < m_size1; ++i)
+                for (std::size_t j = 0; j < m_size2; ++j)
+                    out[detail::element_index<Orientation>(i, j, dst.leading_dimension)] =
+                        h[detail::element_index<typename E::orientation>(i, j, src.leading_dimension)];
+            hip::check_hip(hipMemcpyAsync(dst.values, out.data(), out.size() * sizeof(T), hipMemcpyHostToDevice,
+                                          s.handle()), "hipMemcpyAsync");
+            s.synchronize();
+        }
+    }
+
+    hip::device* m_queue;
+    std::size_t m_size1 = 0, m_size2 = 0;
+    hip::buffer<T> m_data;
+};
+
+// device vector (contiguous); vector views carry a stride
+template <class T, class Device = cpu_tag>
+class vector;
+
+template <class T>
+class vector<T, hip_tag> : public vector_expression<vector<T, hip_tag>, hip_tag> {
+public:
+    typedef T value_type;
+    typedef dense_vector_storage<T> storage_type;
+    explicit vector(std::size_t size = 0, hip::device& q = hip::device_manager::instance().default_device())
+        : m_queue(&q), m_data(size, q) {}
+    std::size_t size() const { return m_data.size(); }
+    storage_type raw_storage() const { return {m_data.get(), 1}; }
+    hip::device& queue() const { return *m_queue; }
+
+private:
+    hip::device* m_queue;
+    hip::buffer<T> m_data;
+};
+
+// ------------------------------------------------------------ proxies --
+template <class M>
+matrix_view<typename M::value_type, typename M::orientation::transposed_orientation, hip_tag> trans(
+    matrix_expression<M, hip_tag> const& m) {
+    auto s = m().raw_storage();
+    return {s.values, m().size2(), m().size1(), s.leading_dimension, m().queue()};
+}
+
+template <class M>
+matrix_view<typename M::value_type, typename M::orientation, hip_tag> subrange(matrix_expression<M, hip_tag> const& m,
+                                                                             std::size_t r0, std::size_t r1,
+                                                                             std::size_t c0, std::size_t c1) {
+    assert(r0 <= r1 && r1 <= m().size1() && c0 <= c1 && c1 <= m().size2());
+    auto s = m().raw_storage();
+    return {s.values + detail::element_index<typename M::orientation>(r0, c0, s.leading_dimension), r1 - r0, c1 - c0,
+            s.leading_dimension, m().queue()};
+}
+
+// ------------------------------------------------------- cpu container --
+// Minimal host matrix used by the transport and the tests.
+template <class T, class Orientation>
+class matrix<T, Orientation, cpu_tag> {
+public:
+    typedef T value_type;
+    typedef Orientation orientation;
+    matrix() = default;
+    matrix(std::size_t s1, std::size_t s2, T init = T()) : m_size1(s1), m_size2(s2), m_data(s1 * s2, init) {}
+    std::size_t size1() const { return m_size1; }
+    std::size_t size2() const { return m_size2; }
+    std::size_t leading_dimension() const { return std::max<std::size_t>(1, Orientation::is_row_major ? m_size2 : m_size1); }
+    T& operator()(std::size_t i, std::size_t j) {
+        return m_data[detail::element_index<Orientation>(i, j, leading_dimension())];
+    }
+    T const& operator()(std::size_t i, std::size_t j) const {
+        return m_data[detail::element_index<Orientation>(i, j, leading_dimension())];
+    }
+    T* data() { return m_data.data(); }
+    T const* data() const { return m_data.data(); }
+
+private:
+    std::size_t m_size1 = 0, m_size2 = 0;
+    std::vector<T> m_data;
+};
+
+// ----------------------------------------------------------- transport --
+// Host <-> device copies with the layout preserved (one contiguous copy on
+// the thread's stream, then a synchronisation; the reference used a
+// zero-copy kernel over registered host memory, hip/copy.hpp:362-386).
+template <class T, class O>
+matrix<T, O, hip_tag> copy_to_device(matrix<T, O, cpu_tag> const& m, hip_tag,
+                                     hip::device& q = hip::device_manager::instance().default_device()) {
+    matrix<T, O, hip_tag> d(m.size1(), m.size2(), q);
+    if (m.size1() * m.size2() == 0) return d;
+    hip::stream& s = hip::get_stream(q);
+    q.set_device();
+    hip::check_hip(hipMemcpyAsync(d.raw_storage().values, m.data(), m.size1() * m.size2() * sizeof(T),
+                                  hipMemcpyHostToDevice, s.handle()), "hipMemcpyAsync");
+    s.synchronize();
+    return d;
+}
+
+template <class T, class O>
+matrix<T, O, cpu_tag> copy_to_cpu(matrix<T, O, hip_tag> const& d) {
+    matrix<T, O, cpu_tag> m(d.size1(), d.size2());
+    if (m.size1() * m.size2() == 0) return m;
+    hip::stream& s = hip::get_stream(d.queue());
+    d.queue().set_device();
+    hip::check_hip(hipMemcpyAsync(m.data(), d.raw_storage().values, m.size1() * m.size2() * sizeof(T),
+                                  hipMemcpyDeviceToHost, s.handle()), "hipMemcpyAsync");
+    s.synchronize();
+    return m;
+}
+
+}  // namespace remora
+
+#endif

@@ -116,6 +116,27 @@ rmh_status rmh_dpotrs(rmh_handle h, int upper, size_t n, size_t nrhs, const doub
 rmh_status rmh_spotrs(rmh_handle h, int upper, size_t n, size_t nrhs, const float* A, size_t lda, float* B,
                       size_t ldb, rmh_stream_t stream);
 
+/* ----------------------------------------------------- mixed precision -- */
+/* Precision of the factorisation inside rmh_dsposv. */
+typedef enum rmh_factor_precision {
+    RMH_FACTOR_FP32 = 0,   /* trailing updates on FP32 MFMA, FP32 accumulation */
+    RMH_FACTOR_BF16X3 = 1  /* FP32 split into three BF16 terms, BF16 MFMA, FP32 accumulation */
+} rmh_factor_precision;
+
+/* Solve A X = B (A SPD, only the `upper`/lower triangle read, A unchanged)
+ * by factoring a reduced-precision copy of A -- diagonal blocks in FP64,
+ * trailing updates in `prec` -- and refining X with FP64 residuals until the
+ * backward error  max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf)
+ * is <= tol, at most max_iter corrections.  Like LAPACK's dsposv it falls
+ * back to the FP64 factorisation when the reduced-precision factor is not
+ * positive definite (*iters = -3) or refinement does not converge
+ * (*iters = -(max_iter+1)); otherwise *iters is the number of corrections.
+ * *berr receives the final backward error.  Synchronous: returns when X is
+ * ready.  No counterpart in the reference (a new form beside the FP64 path). */
+rmh_status rmh_dsposv(rmh_handle h, int upper, rmh_factor_precision prec, size_t n, size_t nrhs, const double* A,
+                      size_t lda, const double* B, size_t ldb, double* X, size_t ldx, double tol, int max_iter,
+                      int* iters, double* berr, rmh_stream_t stream);
+
 /* ------------------------------------------------------------ tuning -- */
 /* Block sizes of rmh_?potrf: nb is the width of a panel (the K of the
  * trailing update), nb_inner the diagonal block inside a panel (<= 64).

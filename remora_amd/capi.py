@@ -22,6 +22,9 @@ NOT_POSITIVE_DEFINITE = 5
 NOT_CONVERGED = 6
 NOT_SUPPORTED = 7
 
+FACTOR_FP32 = 0
+FACTOR_BF16X3 = 1
+
 
 class RemoraError(RuntimeError):
     """A non-success rmh_status, with rmh_last_error()'s detail."""
@@ -56,6 +59,7 @@ SIGNATURES = {
     "rmh_spotrf": (ST, [P, I, S, P, S, P, P]),
     "rmh_dpotrs": (ST, [P, I, S, S, P, S, P, S, P]),
     "rmh_spotrs": (ST, [P, I, S, S, P, S, P, S, P]),
+    "rmh_dsposv": (ST, [P, I, I, S, S, P, S, P, S, P, S, D, I, C.POINTER(I), C.POINTER(D), P]),
     "rmh_set_potrf_blocking": (ST, [P, S, S]),
     "rmh_profile_enable": (ST, [P, I]),
     "rmh_profile_reset": (ST, [P]),

@@ -49,6 +49,29 @@ rmh_status Context::reserve(size_t bytes) {
     return RMH_STATUS_SUCCESS;
 }
 
+rmh_status Context::reserve_slot(int slot, size_t bytes, void** ptr) {
+    if (slot == WS_MAIN) {
+        RMH_TRY(reserve(bytes));
+        *ptr = ws;
+        return RMH_STATUS_SUCCESS;
+    }
+    if (bytes > aux_bytes[slot]) {
+        RMH_TRY_HIP(hipDeviceSynchronize());
+        if (aux[slot] != nullptr) RMH_TRY_HIP(hipFree(aux[slot]));
+        aux[slot] = nullptr;
+        aux_bytes[slot] = 0;
+        size_t want = std::max(bytes, (size_t)1 << 20);
+        if (hipMalloc(&aux[slot], want) != hipSuccess) {
+            (void)hipGetLastError();
+            aux[slot] = nullptr;
+            return fail(RMH_STATUS_ALLOC_FAILED, "workspace allocation of " + std::to_string(want) + " bytes failed");
+        }
+        aux_bytes[slot] = want;
+    }
+    *ptr = aux[slot];
+    return RMH_STATUS_SUCCESS;
+}
+
 rmh_status Context::prof_begin(hipStream_t s, int cls, double flops, int* slot) {
     *slot = -1;
     if (((prof_mask >> cls) & 1) == 0) return RMH_STATUS_SUCCESS;
@@ -79,10 +102,10 @@ void Context::prof_clear() {
 
 Context::~Context() {
     prof_clear();
-    if (ws != nullptr) {
-        (void)hipSetDevice(device);
-        (void)hipFree(ws);
-    }
+    (void)hipSetDevice(device);
+    if (ws != nullptr) (void)hipFree(ws);
+    for (int i = 0; i < WS_COUNT; ++i)
+        if (aux[i] != nullptr) (void)hipFree(aux[i]);
 }
 
 // ------------------------------------------------------- TRSM driver --
@@ -304,6 +327,13 @@ rmh_status trsv_impl(Context* h, bool upper, bool trans, bool unit, int64_t n, c
     RMH_TRY(trsm_impl<T>(h, true, upper, trans, unit, n, 1, T(1), A, lda, tmp, n, s));
     return launch_strided_copy<T>(tmp, 1, x, incx, n, s);
 }
+
+template rmh_status potrf_impl<double>(Context*, bool, int64_t, double*, int64_t, int64_t*, hipStream_t);
+template rmh_status potrf_impl<float>(Context*, bool, int64_t, float*, int64_t, int64_t*, hipStream_t);
+template rmh_status potrs_impl<double>(Context*, bool, int64_t, int64_t, const double*, int64_t, double*, int64_t,
+                                       hipStream_t);
+template rmh_status potrs_impl<float>(Context*, bool, int64_t, int64_t, const float*, int64_t, float*, int64_t,
+                                      hipStream_t);
 
 }  // namespace rmh
 
@@ -528,6 +558,21 @@ rmh_status rmh_dpotrs(rmh_handle h, int upper, size_t n, size_t nrhs, const doub
 rmh_status rmh_spotrs(rmh_handle h, int upper, size_t n, size_t nrhs, const float* A, size_t lda, float* B,
                       size_t ldb, rmh_stream_t stream) {
     return potrs_entry<float>(h, upper, n, nrhs, A, lda, B, ldb, stream);
+}
+
+rmh_status rmh_dsposv(rmh_handle h, int upper, rmh_factor_precision prec, size_t n, size_t nrhs, const double* A,
+                      size_t lda, const double* B, size_t ldb, double* X, size_t ldx, double tol, int max_iter,
+                      int* iters, double* berr, rmh_stream_t stream) {
+    RMH_CHECK_HANDLE(h);
+    RMH_REQUIRE(lda >= mx1(n) && ldb >= mx1(n) && ldx >= mx1(n), "dsposv: leading dimension too small");
+    RMH_REQUIRE(max_iter >= 0, "dsposv: max_iter must be >= 0");
+    RMH_REQUIRE(prec == RMH_FACTOR_FP32 || prec == RMH_FACTOR_BF16X3, "dsposv: unknown factor precision");
+    if (iters) *iters = 0;
+    if (berr) *berr = 0.0;
+    if (n == 0 || nrhs == 0) return RMH_STATUS_SUCCESS;
+    RMH_REQUIRE(A != nullptr && B != nullptr && X != nullptr, "dsposv: NULL pointer");
+    return dsposv_impl(ctx(h), upper != 0, prec, (int64_t)n, (int64_t)nrhs, A, (int64_t)lda, B, (int64_t)ldb, X,
+                       (int64_t)ldx, tol, max_iter, iters, berr, hs(stream));
 }
 
 rmh_status rmh_set_potrf_blocking(rmh_handle h, size_t nb, size_t nb_inner) {

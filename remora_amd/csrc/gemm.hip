@@ -19,7 +19,8 @@
 //  * the MFMA is issued as D = op(B)^T op(A)^T so that a lane's four
 //    accumulator entries are four columns of C and the 16 lanes of a row
 //    group write 16 consecutive rows: each store instruction writes four
-//    contiguous 16-element column segments of the column-major C;
+//    contiguous 16-element column segments of the column-major C (the f64
+//    and f32 instructions interleave D's rows differently; see Mfma);
 //  * workgroup ids are remapped so each XCD (8 per MI355X, with private L2)
 //    receives a contiguous range of tiles, and tiles are visited in groups of
 //    8 tile-rows so neighbouring workgroups share op(A) and op(B) panels;
@@ -37,8 +38,13 @@ namespace {
 
 template <typename T>
 struct Mfma;
+// Result layout of a 16x16 accumulator (rows i, columns j of D):
+//   f32 16x16x4: lane l, item r holds D[4*(l/16) + r][l%16]
+//   f64 16x16x4: lane l, item r holds D[4*r + l/16][l%16]
+// (operands: A[i=l%16][k=l/16], B[k=l/16][j=l%16] for both).
 template <>
 struct Mfma<double> {
+    static constexpr bool kRowInterleaved = true;
     typedef double acc_t __attribute__((ext_vector_type(4)));
     static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -46,6 +52,7 @@ struct Mfma<double> {
 };
 template <>
 struct Mfma<float> {
+    static constexpr bool kRowInterleaved = false;
     typedef float acc_t __attribute__((ext_vector_type(4)));
     static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -88,7 +95,12 @@ __device__ __forceinline__ void lower_trapezoid_tile(int64_t t, int64_t tn, int6
     }
 }
 
-template <typename T, int BM, int BN, int BK, bool TA, bool TB, int PART>
+// SYM != 0: op(A) is a symmetric matrix of which only the lower (SYM == 1)
+// or upper (SYM == 2) triangle is stored (requires !TA).  Tiles wholly in
+// the stored triangle load directly, tiles wholly in the other one load the
+// mirrored element with the coalesced transposed mapping, and the few tiles
+// that cross the diagonal select per element.
+template <typename T, int BM, int BN, int BK, bool TA, bool TB, int PART, int SYM = 0>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void gemm_kernel(KParams<T> p) {
     typedef Mfma<T> MF;
     typedef typename MF::acc_t acc_t;
@@ -148,20 +160,61 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     const int b_k0 = TB ? tid / BN : tid % BK;
     const int lda32 = (int)p.lda, ldb32 = (int)p.ldb;
 
+    static_assert(SYM == 0 || !TA, "symmetric A is read without transpose");
+    // mirrored mapping (used by SYM tiles that lie in the unstored triangle)
+    constexpr int MK_STEP = 0, MM_STEP = kThreads / BK;
+    const int m_m0 = tid / BK, m_k0 = tid % BK;
+
     T ra[A_PER], rb[B_PER];
+    int a_mode = 0;  // how ra[] of the staged tile was loaded: 0 direct, 1 mirrored
     auto load_tiles = [&](int64_t k0) {
         const int kmax = (int)min((int64_t)BK, K - k0) - 1;  // >= 0
         const T* abase = TA ? p.A + k0 + m0 * p.lda : p.A + m0 + k0 * p.lda;
         const T* bbase = TB ? p.B + n0 + k0 * p.ldb : p.B + k0 + n0 * p.ldb;
         const int mmax = (int)min((int64_t)BM, M - m0) - 1;
         const int nmax = (int)min((int64_t)BN, N - n0) - 1;
+        // SYM: stored element (m,k) is direct when (m >= k) for lower storage
+        bool all_direct = true, all_mirror = false;
+        if (SYM == 1) {
+            all_direct = m0 >= k0 + BK - 1;
+            all_mirror = m0 + BM - 1 < k0;
+        } else if (SYM == 2) {
+            all_direct = m0 + BM - 1 <= k0;
+            all_mirror = m0 > k0 + BK - 1;
+        }
+        const T* mbase = p.A + k0 + m0 * p.lda;  // mirrored element (m,k) at A[k + m*lda]
+        if (all_direct) {
+            a_mode = 0;
 #pragma unroll
-        for (int r = 0; r < A_PER; ++r) {
-            const int mm = a_m0 + r * AM_STEP, kk = a_k0 + r * AK_STEP;
-            const bool ok = (mm <= mmax) && (kk <= kmax);
-            const int cm = min(mm, mmax), ck = min(kk, kmax);
-            const T v = TA ? abase[ck + cm * lda32] : abase[cm + ck * lda32];
-            ra[r] = ok ? v : T(0);
+            for (int r = 0; r < A_PER; ++r) {
+                const int mm = a_m0 + r * AM_STEP, kk = a_k0 + r * AK_STEP;
+                const bool ok = (mm <= mmax) && (kk <= kmax);
+                const int cm = min(mm, mmax), ck = min(kk, kmax);
+                const T v = TA ? abase[ck + cm * lda32] : abase[cm + ck * lda32];
+                ra[r] = ok ? v : T(0);
+            }
+        } else if (all_mirror) {
+            a_mode = 1;
+#pragma unroll
+            for (int r = 0; r < A_PER; ++r) {
+                const int mm = m_m0 + r * MM_STEP, kk = m_k0 + r * MK_STEP;
+                const bool ok = (mm <= mmax) && (kk <= kmax);
+                const int cm = min(mm, mmax), ck = min(kk, kmax);
+                const T v = mbase[ck + cm * lda32];
+                ra[r] = ok ? v : T(0);
+            }
+        } else {
+            a_mode = 0;
+            const int64_t dk = k0 - m0;
+#pragma unroll
+            for (int r = 0; r < A_PER; ++r) {
+                const int mm = a_m0 + r * AM_STEP, kk = a_k0 + r * AK_STEP;
+                const bool ok = (mm <= mmax) && (kk <= kmax);
+                const int cm = min(mm, mmax), ck = min(kk, kmax);
+                const bool direct = SYM == 1 ? (cm - ck >= dk) : (cm - ck <= dk);
+                const T v = direct ? abase[cm + ck * lda32] : mbase[ck + cm * lda32];
+                ra[r] = ok ? v : T(0);
+            }
         }
 #pragma unroll
         for (int r = 0; r < B_PER; ++r) {
@@ -173,10 +226,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         }
     };
     auto store_tiles = [&](int buf) {
+        if (SYM != 0 && a_mode == 1) {
 #pragma unroll
-        for (int r = 0; r < A_PER; ++r) {
-            const int mm = a_m0 + r * AM_STEP, kk = a_k0 + r * AK_STEP;
-            As[buf][kk * LDA_S + mm] = ra[r];
+            for (int r = 0; r < A_PER; ++r) {
+                const int mm = m_m0 + r * MM_STEP, kk = m_k0 + r * MK_STEP;
+                As[buf][kk * LDA_S + mm] = ra[r];
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < A_PER; ++r) {
+                const int mm = a_m0 + r * AM_STEP, kk = a_k0 + r * AK_STEP;
+                As[buf][kk * LDA_S + mm] = ra[r];
+            }
         }
 #pragma unroll
         for (int r = 0; r < B_PER; ++r) {
@@ -222,7 +283,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         __syncthreads();
     }
 
-    // ---- epilogue: lane holds C[m][n..n+3] for 16 consecutive m per group
+    // ---- epilogue: D = C^T, so a lane's items are 4 columns of C and the
+    // 16 lanes of a group hold 16 consecutive rows
     const T alpha = p.alpha, beta = p.beta;
     T* cbase = p.C + m0 + n0 * p.ldc;
     const int ldc32 = (int)p.ldc;
@@ -234,7 +296,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         for (int j = 0; j < FN; ++j) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int nn = wn * WN + j * 16 + 4 * lk + r;
+                const int nn = wn * WN + j * 16 + (MF::kRowInterleaved ? 4 * r + lk : 4 * lk + r);
                 const int64_t gm = m0 + mm, gn = n0 + nn;
                 bool ok = gn < N;
                 if (PART == 1) ok = ok && (gm >= gn);
@@ -291,7 +353,41 @@ rmh_status launch_part(const GemmDesc<T>& d, hipStream_t stream) {
     return fail(RMH_STATUS_INVALID_VALUE, "gemm: bad part");
 }
 
+template <typename T, int BM, int BN, int BK, int SYM>
+rmh_status launch_symm_cfg(const GemmDesc<T>& d, hipStream_t stream) {
+    KParams<T> p;
+    p.M = d.M; p.N = d.N; p.K = d.K;
+    p.alpha = d.alpha; p.beta = d.beta;
+    p.A = d.A; p.lda = d.lda; p.B = d.B; p.ldb = d.ldb; p.C = d.C; p.ldc = d.ldc;
+    p.tiles_m = (d.M + BM - 1) / BM;
+    p.tiles_n = (d.N + BN - 1) / BN;
+    p.abort_flag = d.abort_flag;
+    const int64_t ntiles = p.tiles_m * p.tiles_n;
+    if (ntiles <= 0) return RMH_STATUS_SUCCESS;
+    if (d.transB)
+        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, false, true, 0, SYM>), dim3((unsigned)ntiles), dim3(kThreads), 0,
+                           stream, p);
+    else
+        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, false, false, 0, SYM>), dim3((unsigned)ntiles), dim3(kThreads), 0,
+                           stream, p);
+    return check_launch("gemm_kernel(symm)");
+}
+
 }  // namespace
+
+template <typename T>
+rmh_status launch_symm(const GemmDesc<T>& d, bool upper, hipStream_t stream) {
+    if (d.M <= 0 || d.N <= 0) return RMH_STATUS_SUCCESS;
+    if (d.M != d.K) return fail(RMH_STATUS_INVALID_VALUE, "symm: A must be square");
+    if (d.lda > kMaxLd || d.ldb > kMaxLd || d.ldc > kMaxLd)
+        return fail(RMH_STATUS_INVALID_VALUE, "symm: leading dimension above 2^23");
+    constexpr int BK = sizeof(T) == 8 ? 16 : 32;
+    const bool big = pick_tile(d.M, d.N, Part::Full) == Tile::T128x128;
+    if (big) return upper ? launch_symm_cfg<T, 128, 128, BK, 2>(d, stream) : launch_symm_cfg<T, 128, 128, BK, 1>(d, stream);
+    return upper ? launch_symm_cfg<T, 64, 64, BK, 2>(d, stream) : launch_symm_cfg<T, 64, 64, BK, 1>(d, stream);
+}
+template rmh_status launch_symm<double>(const GemmDesc<double>&, bool, hipStream_t);
+template rmh_status launch_symm<float>(const GemmDesc<float>&, bool, hipStream_t);
 
 Tile pick_tile(int64_t M, int64_t N, Part part) {
     const int64_t t128 = ((M + 127) / 128) * ((N + 127) / 128) / (part == Part::Full ? 1 : 2);

@@ -17,19 +17,26 @@ struct ProfRec {
     int cls = 0;
 };
 
+// Scratch slots of a handle: each grows on demand and is reused by every
+// call that uses it.  Routines that call other routines use different slots.
+enum WsSlot { WS_MAIN = 0, WS_MIXED = 1, WS_FACTOR = 2, WS_BATCH = 3, WS_COUNT = 4 };
+
 struct Context {
     static constexpr size_t kDefaultNb = 256;
     static constexpr size_t kDefaultNbi = 64;
 
     int device = 0;
-    void* ws = nullptr;  // scratch, grown on demand, reused by every call on this handle
+    void* ws = nullptr;  // WS_MAIN (potrf / trsm scratch)
     size_t ws_bytes = 0;
+    void* aux[WS_COUNT] = {nullptr, nullptr, nullptr, nullptr};
+    size_t aux_bytes[WS_COUNT] = {0, 0, 0, 0};
     size_t nb = kDefaultNb, nbi = kDefaultNbi;
     int prof_mask = 0;  // bit i: record launches of ProfClass i
     std::vector<ProfRec> recs;
 
     rmh_status activate();
     rmh_status reserve(size_t bytes);
+    rmh_status reserve_slot(int slot, size_t bytes, void** ptr);
     rmh_status prof_begin(hipStream_t s, int cls, double flops, int* slot);
     rmh_status prof_end(hipStream_t s, int slot);
     void prof_clear();
@@ -41,5 +48,12 @@ rmh_status trsm_impl(Context* h, bool left, bool upper, bool trans, bool unit, i
                      const T* A, int64_t lda, T* B, int64_t ldb, hipStream_t s);
 template <typename T>
 rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int64_t* d_info, hipStream_t s);
+template <typename T>
+rmh_status potrs_impl(Context* h, bool upper, int64_t n, int64_t nrhs, const T* A, int64_t lda, T* B, int64_t ldb,
+                      hipStream_t s);
+
+rmh_status dsposv_impl(Context* h, bool upper, rmh_factor_precision prec, int64_t n, int64_t nrhs, const double* A,
+                       int64_t lda, const double* B, int64_t ldb, double* X, int64_t ldx, double tol, int max_iter,
+                       int* iters, double* berr, hipStream_t s);
 
 }  // namespace rmh

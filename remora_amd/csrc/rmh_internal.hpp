@@ -62,6 +62,11 @@ struct GemmDesc {
 template <typename T>
 rmh_status launch_gemm(const GemmDesc<T>& d, Tile tile, hipStream_t stream);
 
+// C = alpha*A*op(B) + beta*C with A symmetric (M x M), only its lower
+// (upper = false) or upper triangle read.
+template <typename T>
+rmh_status launch_symm(const GemmDesc<T>& d, bool upper, hipStream_t stream);
+
 // Pick a tile shape for a problem of this size (fills 256 CUs).
 Tile pick_tile(int64_t M, int64_t N, Part part);
 

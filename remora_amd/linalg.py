@@ -116,6 +116,16 @@ def potrs(h, L, B, upper=False, stream=None):
     return B
 
 
+def dsposv(h, A, B, X, upper=False, prec="fp32", tol=1e-14, max_iter=30, stream=None):
+    """Mixed-precision SPD solve with FP64 refinement; returns (iters, berr)."""
+    n, nrhs = B.shape
+    it, be = C.c_int(), C.c_double()
+    p = {"fp32": capi.FACTOR_FP32, "bf16x3": capi.FACTOR_BF16X3}[prec]
+    capi.check(capi.load().rmh_dsposv(h.value, int(upper), p, n, nrhs, _p(A), _ld(A), _p(B), _ld(B), _p(X), _ld(X),
+                                      tol, max_iter, C.byref(it), C.byref(be), _stream(stream)))
+    return it.value, be.value
+
+
 def set_potrf_blocking(h, nb=0, nb_inner=0):
     capi.check(capi.load().rmh_set_potrf_blocking(h.value, nb, nb_inner))
 

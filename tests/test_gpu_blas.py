@@ -59,7 +59,8 @@ def test_dgemm_beta_zero_ignores_nan(handle):
     la.gemm(handle, dev(A), dev(A), dC, alpha=1.0, beta=0.0)
     got = host(dC)
     assert np.all(np.isfinite(got))
-    np.testing.assert_allclose(got, A @ A, rtol=1e-12, atol=1e-12 * np.abs(A) @ np.abs(A))
+    ok, worst = componentwise_ok(got, A @ A, np.abs(A) @ np.abs(A), TOL["d"])
+    assert ok, worst
 
 
 @pytest.mark.parametrize("ta", [False, True])
@@ -125,8 +126,11 @@ def test_dsyrk_random(handle, upper, trans, nk):
 
 
 def test_ssyrk_reference_hip_fixture(handle):
-    """Test/hip_syrk.cpp:17-98 (float, dims 936, K 1039): closed form in
-    double on the host, 1e-4 percent."""
+    """Test/hip_syrk.cpp:17-98 (float, dims 936, K 1039).  The reference
+    compares against a float inner product on the CPU at 1e-4 percent, i.e.
+    two FP32 accumulations of 1039 terms; against the exact (double) closed
+    form the FP32 rounding of such a sum reaches ~sqrt(K)*u, so the bound
+    here is 1e-5 relative (0.15 of the worst-case gamma_K)."""
     import torch
     from remora_amd import linalg as la
     arg = fixtures.syrk_arg(936, 1039)
@@ -136,7 +140,7 @@ def test_ssyrk_reference_hip_fixture(handle):
     a32 = arg.astype(np.float32).astype(np.float64)
     ref = 2.0 * a32 @ a32.T + 3.0
     mask = np.tril(np.ones((936, 936), bool))
-    np.testing.assert_allclose(got[mask], ref[mask], rtol=1e-6)
+    np.testing.assert_allclose(got[mask], ref[mask], rtol=1e-5)
 
 
 CASES = [(left, upper, trans, unit) for left in (True, False) for upper in (False, True)
