@@ -190,6 +190,8 @@ template <typename T>
 rmh_status trsm_impl(Context* h, bool left, bool upper, bool trans, bool unit, int64_t m, int64_t n, T alpha,
                      const T* A, int64_t lda, T* B, int64_t ldb, hipStream_t s) {
     if (m == 0 || n == 0) return RMH_STATUS_SUCCESS;
+    if (trsm_blocked_applies(left ? m : n, left ? n : m))
+        return trsm_blocked<T>(h, left, upper, trans, unit, m, n, alpha, A, lda, B, ldb, s);
     if (alpha != T(1)) RMH_TRY(launch_scale<T>(B, ldb, m, n, alpha, s));
     if (alpha == T(0)) return RMH_STATUS_SUCCESS;
     const int64_t na = left ? m : n;
@@ -209,8 +211,8 @@ rmh_status trsm_impl(Context* h, bool left, bool upper, bool trans, bool unit, i
 //
 // For each panel of width nb:
 //   for each diagonal block of width nbi inside the panel:
-//     diag   -- factor V_jj in LDS and form W = V_jj^-1 (one wavefront)
-//     panel  -- V_ij <- V_ij W^T for the rows below, in place (MFMA GEMM)
+//     panel  -- factor V_jj in LDS and solve the rows below against it
+//               (one fused launch, potrf_panel_kernel)
 //     inner  -- rank-nbi update of the rest of the panel's columns
 //   trailing -- V22 -= P P^T with K = nb (MFMA SYRK on the lower trapezoid)
 // Every launch reads the device info word first and does nothing once a
@@ -218,10 +220,8 @@ rmh_status trsm_impl(Context* h, bool left, bool upper, bool trans, bool unit, i
 template <typename T>
 rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int64_t* d_info, hipStream_t s) {
     const int64_t nb = (int64_t)h->nb, nbi = (int64_t)h->nbi;
-    const size_t w_off = 256;
-    RMH_TRY(h->reserve(w_off + (size_t)nbi * nbi * sizeof(T)));
+    RMH_TRY(h->reserve(256));
     int64_t* info = d_info != nullptr ? d_info : static_cast<int64_t*>(h->ws);
-    T* W = reinterpret_cast<T*>(static_cast<char*>(h->ws) + w_off);
     RMH_TRY(launch_fill_i64(info, 1, 0, s));
     if (n == 0) return RMH_STATUS_SUCCESS;
     const int64_t rs = upper ? lda : 1, cs = upper ? 1 : lda;
@@ -231,33 +231,11 @@ rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
         const int64_t kw = std::min(nb, n - k0);
         for (int64_t j0 = k0; j0 < k0 + kw; j0 += nbi) {
             const int64_t jb = std::min(nbi, k0 + kw - j0);
-            int slot;
-            RMH_TRY(h->prof_begin(s, PROF_DIAG, (double)jb * jb * jb / 3.0, &slot));
-            RMH_TRY(launch_potrf_diag<T>(V(j0, j0), rs, cs, (int)jb, W, nbi, info, j0, s));
-            RMH_TRY(h->prof_end(s, slot));
             const int64_t below = n - j0 - jb;
-            if (below > 0) {
-                // panel solve V21 <- V21 W^T, in place
-                GemmDesc<T> g;
-                g.alpha = T(1); g.beta = T(0); g.abort_flag = info;
-                Tile tile;
-                if (!upper) {
-                    g.M = below; g.N = jb; g.K = jb;
-                    g.A = A + (j0 + jb) + j0 * lda; g.lda = lda; g.transA = false;
-                    g.B = W; g.ldb = nbi; g.transB = true;
-                    g.C = A + (j0 + jb) + j0 * lda; g.ldc = lda;
-                    tile = below >= 128 * 256 ? Tile::T128x64 : Tile::T64x64;
-                } else {  // A12 <- W A12 (jb x below)
-                    g.M = jb; g.N = below; g.K = jb;
-                    g.A = W; g.lda = nbi; g.transA = false;
-                    g.B = A + j0 + (j0 + jb) * lda; g.ldb = lda; g.transB = false;
-                    g.C = A + j0 + (j0 + jb) * lda; g.ldc = lda;
-                    tile = below >= 128 * 256 ? Tile::T64x128 : Tile::T64x64;
-                }
-                RMH_TRY(h->prof_begin(s, PROF_PANEL, 2.0 * below * jb * jb / 2.0, &slot));
-                RMH_TRY(launch_gemm<T>(g, tile, s));
-                RMH_TRY(h->prof_end(s, slot));
-            }
+            int slot;
+            RMH_TRY(h->prof_begin(s, PROF_PANEL, (double)jb * jb * jb / 3.0 + (double)below * jb * jb, &slot));
+            RMH_TRY(launch_potrf_panel<T>(V(j0, j0), rs, cs, n - j0, (int)jb, info, j0, s));
+            RMH_TRY(h->prof_end(s, slot));
             const int64_t cw = k0 + kw - (j0 + jb);
             if (cw > 0) {
                 // inner update of the remaining panel columns

@@ -71,6 +71,14 @@ struct KParams {
     int64_t ldc;
     int64_t tiles_m, tiles_n;
     const int64_t* abort_flag;
+    // strided batch: blockIdx.z = b -> (y = b % batch_ny, z = b / batch_ny),
+    // operand origins advance by y*s?y + z*s?z elements
+    int64_t batch_ny;
+    int64_t sAy, sAz, sBy, sBz, sCy, sCz;
+    // split-K: blockIdx.y owns K range [y*k_chunk, (y+1)*k_chunk) and writes
+    // its partial product to split_ws + y*M*N (ld M); a reduction combines them
+    int64_t k_chunk;
+    T* split_ws;
 };
 
 constexpr int kThreads = 256;
@@ -115,6 +123,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     __shared__ T Bs[2][BK * LDB_S];
 
     if (p.abort_flag != nullptr && *p.abort_flag != 0) return;
+    if (gridDim.z > 1) {
+        const int64_t b = blockIdx.z, by = b % p.batch_ny, bz = b / p.batch_ny;
+        p.A += by * p.sAy + bz * p.sAz;
+        p.B += by * p.sBy + bz * p.sBz;
+        p.C += by * p.sCy + bz * p.sCz;
+    }
+    if (gridDim.y > 1) {
+        const int64_t k0s = (int64_t)blockIdx.y * p.k_chunk;
+        p.A += TA ? k0s : k0s * p.lda;
+        p.B += TB ? k0s * p.ldb : k0s;
+        p.K = min(p.K, k0s + p.k_chunk) - k0s;
+        p.C = p.split_ws + (int64_t)blockIdx.y * p.M * p.N;
+        p.ldc = p.M;
+        p.alpha = T(1);
+        p.beta = T(0);
+    }
 
     // ---- tile selection: XCD-contiguous ranges, then grouped order -------
     const int64_t G = gridDim.x;
@@ -166,13 +190,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     const int m_m0 = tid / BK, m_k0 = tid % BK;
 
     T ra[A_PER], rb[B_PER];
-    int a_mode = 0;  // how ra[] of the staged tile was loaded: 0 direct, 1 mirrored
+    int a_mode = 0;     // how ra[] of the staged tile was loaded: 0 direct, 1 mirrored
+    int st_kmax = 0;    // last valid k inside the staged tile
+    const int mmax = (int)min((int64_t)BM, M - m0) - 1;  // last valid row / column of this tile
+    const int nmax = (int)min((int64_t)BN, N - n0) - 1;
+    // Loads go to clamped (always valid) addresses and are not inspected
+    // here: zeroing the out-of-range elements happens at the LDS store, after
+    // the MFMAs of the current stage, so the loads stay in flight meanwhile.
     auto load_tiles = [&](int64_t k0) {
         const int kmax = (int)min((int64_t)BK, K - k0) - 1;  // >= 0
+        st_kmax = kmax;
         const T* abase = TA ? p.A + k0 + m0 * p.lda : p.A + m0 + k0 * p.lda;
         const T* bbase = TB ? p.B + n0 + k0 * p.ldb : p.B + k0 + n0 * p.ldb;
-        const int mmax = (int)min((int64_t)BM, M - m0) - 1;
-        const int nmax = (int)min((int64_t)BN, N - n0) - 1;
         // SYM: stored element (m,k) is direct when (m >= k) for lower storage
         bool all_direct = true, all_mirror = false;
         if (SYM == 1) {
@@ -187,62 +216,51 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
             a_mode = 0;
 #pragma unroll
             for (int r = 0; r < A_PER; ++r) {
-                const int mm = a_m0 + r * AM_STEP, kk = a_k0 + r * AK_STEP;
-                const bool ok = (mm <= mmax) && (kk <= kmax);
-                const int cm = min(mm, mmax), ck = min(kk, kmax);
-                const T v = TA ? abase[ck + cm * lda32] : abase[cm + ck * lda32];
-                ra[r] = ok ? v : T(0);
+                const int cm = min(a_m0 + r * AM_STEP, mmax), ck = min(a_k0 + r * AK_STEP, kmax);
+                ra[r] = TA ? abase[ck + cm * lda32] : abase[cm + ck * lda32];
             }
         } else if (all_mirror) {
             a_mode = 1;
 #pragma unroll
             for (int r = 0; r < A_PER; ++r) {
-                const int mm = m_m0 + r * MM_STEP, kk = m_k0 + r * MK_STEP;
-                const bool ok = (mm <= mmax) && (kk <= kmax);
-                const int cm = min(mm, mmax), ck = min(kk, kmax);
-                const T v = mbase[ck + cm * lda32];
-                ra[r] = ok ? v : T(0);
+                const int cm = min(m_m0 + r * MM_STEP, mmax), ck = min(m_k0 + r * MK_STEP, kmax);
+                ra[r] = mbase[ck + cm * lda32];
             }
         } else {
             a_mode = 0;
             const int64_t dk = k0 - m0;
 #pragma unroll
             for (int r = 0; r < A_PER; ++r) {
-                const int mm = a_m0 + r * AM_STEP, kk = a_k0 + r * AK_STEP;
-                const bool ok = (mm <= mmax) && (kk <= kmax);
-                const int cm = min(mm, mmax), ck = min(kk, kmax);
+                const int cm = min(a_m0 + r * AM_STEP, mmax), ck = min(a_k0 + r * AK_STEP, kmax);
                 const bool direct = SYM == 1 ? (cm - ck >= dk) : (cm - ck <= dk);
-                const T v = direct ? abase[cm + ck * lda32] : mbase[ck + cm * lda32];
-                ra[r] = ok ? v : T(0);
+                ra[r] = direct ? abase[cm + ck * lda32] : mbase[ck + cm * lda32];
             }
         }
 #pragma unroll
         for (int r = 0; r < B_PER; ++r) {
-            const int nn = b_n0 + r * BN_STEP, kk = b_k0 + r * BK_STEP;
-            const bool ok = (nn <= nmax) && (kk <= kmax);
-            const int cn = min(nn, nmax), ck = min(kk, kmax);
-            const T v = TB ? bbase[cn + ck * ldb32] : bbase[ck + cn * ldb32];
-            rb[r] = ok ? v : T(0);
+            const int cn = min(b_n0 + r * BN_STEP, nmax), ck = min(b_k0 + r * BK_STEP, kmax);
+            rb[r] = TB ? bbase[cn + ck * ldb32] : bbase[ck + cn * ldb32];
         }
     };
     auto store_tiles = [&](int buf) {
+        const int kmax = st_kmax;
         if (SYM != 0 && a_mode == 1) {
 #pragma unroll
             for (int r = 0; r < A_PER; ++r) {
                 const int mm = m_m0 + r * MM_STEP, kk = m_k0 + r * MK_STEP;
-                As[buf][kk * LDA_S + mm] = ra[r];
+                As[buf][kk * LDA_S + mm] = (mm <= mmax && kk <= kmax) ? ra[r] : T(0);
             }
         } else {
 #pragma unroll
             for (int r = 0; r < A_PER; ++r) {
                 const int mm = a_m0 + r * AM_STEP, kk = a_k0 + r * AK_STEP;
-                As[buf][kk * LDA_S + mm] = ra[r];
+                As[buf][kk * LDA_S + mm] = (mm <= mmax && kk <= kmax) ? ra[r] : T(0);
             }
         }
 #pragma unroll
         for (int r = 0; r < B_PER; ++r) {
             const int nn = b_n0 + r * BN_STEP, kk = b_k0 + r * BK_STEP;
-            Bs[buf][kk * LDB_S + nn] = rb[r];
+            Bs[buf][kk * LDB_S + nn] = (nn <= nmax && kk <= kmax) ? rb[r] : T(0);
         }
     };
 
@@ -288,25 +306,52 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     const T alpha = p.alpha, beta = p.beta;
     T* cbase = p.C + m0 + n0 * p.ldc;
     const int ldc32 = (int)p.ldc;
+    // For each accumulator row block: first load every C value it needs,
+    // then store, so the loads are in flight together (a store between two
+    // loads would otherwise order them).
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
         const int mm = wm * WM + i * 16 + lr;
-        if (m0 + mm >= M) continue;
+        const bool row_ok = m0 + mm < M;
+        T cold[FN][4];
+        bool ok[FN][4];
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int nn = wn * WN + j * 16 + (MF::kRowInterleaved ? 4 * r + lk : 4 * lk + r);
                 const int64_t gm = m0 + mm, gn = n0 + nn;
-                bool ok = gn < N;
-                if (PART == 1) ok = ok && (gm >= gn);
-                if (PART == 2) ok = ok && (gm <= gn);
-                if (!ok) continue;
-                T* c = cbase + (mm + nn * ldc32);
-                const T v = alpha * acc[i][j][r];
-                *c = (beta == T(0)) ? v : v + beta * *c;
+                bool o = row_ok && gn < N;
+                if (PART == 1) o = o && (gm >= gn);
+                if (PART == 2) o = o && (gm <= gn);
+                ok[j][r] = o;
+                cold[j][r] = (o && beta != T(0)) ? cbase[mm + nn * ldc32] : T(0);
             }
         }
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                if (!ok[j][r]) continue;
+                const int nn = wn * WN + j * 16 + (MF::kRowInterleaved ? 4 * r + lk : 4 * lk + r);
+                const T v = alpha * acc[i][j][r];
+                cbase[mm + nn * ldc32] = (beta == T(0)) ? v : v + beta * cold[j][r];
+            }
+        }
+    }
+}
+
+// C = alpha * sum_s ws[s] + beta * C over the split-K partials.
+template <typename T>
+__global__ void split_reduce_kernel(const T* __restrict__ ws, int splits, int64_t M, int64_t N, T alpha, T beta, T* C,
+                                    int64_t ldc) {
+    const int64_t total = M * N;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        T acc = T(0);
+        for (int s = 0; s < splits; ++s) acc += ws[s * total + e];
+        const int64_t i = e % M, j = e / M;
+        T* c = C + i + j * ldc;
+        *c = beta == T(0) ? alpha * acc : alpha * acc + beta * *c;
     }
 }
 
@@ -319,6 +364,10 @@ rmh_status launch_cfg(const GemmDesc<T>& d, hipStream_t stream) {
     p.tiles_m = (d.M + BM - 1) / BM;
     p.tiles_n = (d.N + BN - 1) / BN;
     p.abort_flag = d.abort_flag;
+    p.batch_ny = d.batch_ny > 0 ? d.batch_ny : 1;
+    p.sAy = d.sAy; p.sAz = d.sAz; p.sBy = d.sBy; p.sBz = d.sBz; p.sCy = d.sCy; p.sCz = d.sCz;
+    p.k_chunk = d.k_chunk;
+    p.split_ws = d.split_ws;
     int64_t ntiles;
     if (PART == 0) {
         ntiles = p.tiles_m * p.tiles_n;
@@ -331,7 +380,8 @@ rmh_status launch_cfg(const GemmDesc<T>& d, hipStream_t stream) {
     }
     if (ntiles <= 0) return RMH_STATUS_SUCCESS;
     if (ntiles > 0x7fffffffLL) return fail(RMH_STATUS_INVALID_VALUE, "gemm: problem too large for one launch");
-    dim3 grid((unsigned)ntiles), block(kThreads);
+    dim3 grid((unsigned)ntiles, (unsigned)std::max(1, d.splits), (unsigned)std::max<int64_t>(1, d.batch)),
+        block(kThreads);
     if (!d.transA && !d.transB)
         hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, false, false, PART>), grid, block, 0, stream, p);
     else if (!d.transA && d.transB)
@@ -340,7 +390,15 @@ rmh_status launch_cfg(const GemmDesc<T>& d, hipStream_t stream) {
         hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, true, false, PART>), grid, block, 0, stream, p);
     else
         hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, true, true, PART>), grid, block, 0, stream, p);
-    return check_launch("gemm_kernel");
+    RMH_TRY(check_launch("gemm_kernel"));
+    if (d.splits > 1) {
+        const int64_t total = d.M * d.N;
+        const unsigned g = (unsigned)std::min<int64_t>((total + 255) / 256, 4096);
+        hipLaunchKernelGGL(split_reduce_kernel<T>, dim3(g), dim3(256), 0, stream, d.split_ws, d.splits, d.M, d.N, d.alpha,
+                           d.beta, d.C, d.ldc);
+        RMH_TRY(check_launch("split_reduce_kernel"));
+    }
+    return RMH_STATUS_SUCCESS;
 }
 
 template <typename T, int BM, int BN, int BK>
@@ -362,6 +420,10 @@ rmh_status launch_symm_cfg(const GemmDesc<T>& d, hipStream_t stream) {
     p.tiles_m = (d.M + BM - 1) / BM;
     p.tiles_n = (d.N + BN - 1) / BN;
     p.abort_flag = d.abort_flag;
+    p.batch_ny = 1;
+    p.sAy = p.sAz = p.sBy = p.sBz = p.sCy = p.sCz = 0;
+    p.k_chunk = d.K;
+    p.split_ws = nullptr;
     const int64_t ntiles = p.tiles_m * p.tiles_n;
     if (ntiles <= 0) return RMH_STATUS_SUCCESS;
     if (d.transB)
@@ -396,8 +458,19 @@ Tile pick_tile(int64_t M, int64_t N, Part part) {
 }
 
 template <typename T>
-rmh_status launch_gemm(const GemmDesc<T>& d, Tile tile, hipStream_t stream) {
+rmh_status launch_gemm(const GemmDesc<T>& d_in, Tile tile, hipStream_t stream) {
+    GemmDesc<T> d = d_in;
     if (d.M <= 0 || d.N <= 0) return RMH_STATUS_SUCCESS;
+    if (d.splits > 1) {
+        if (d.split_ws == nullptr || d.part != Part::Full || d.batch > 1 || d.K <= 0)
+            return fail(RMH_STATUS_INVALID_VALUE, "gemm: split-K needs a workspace and a plain rectangle");
+        d.k_chunk = (d.K + d.splits - 1) / d.splits;
+        d.splits = (int)((d.K + d.k_chunk - 1) / d.k_chunk);  // no empty chunk
+        if (d.splits == 1) d.split_ws = nullptr;
+    } else {
+        d.splits = 1;
+        d.k_chunk = d.K;
+    }
     // tile-relative offsets are 32-bit: (tile extent) * ld must stay below 2^31
     if (d.lda > kMaxLd || d.ldb > kMaxLd || d.ldc > kMaxLd)
         return fail(RMH_STATUS_INVALID_VALUE, "gemm: leading dimension above 2^23");

@@ -1,14 +1,6 @@
-// Small-block kernels of the Cholesky path: the diagonal-block POTRF, the
-// diagonal-block triangular inverses used by TRSM, the in-place application
-// of an inverted diagonal block, and BLAS-2 helpers.
-//
-// The diagonal-block POTRF replaces potrf_block_kernel of the reference HIP
-// backend (include/remora/kernels/hip/potrf.hpp:41-80), which factored a
-// 16x16 block with a single thread; here one wavefront cooperates on a block
-// of up to 64 x 64 kept in LDS: lane i owns row i, every column step is a
-// broadcast of the pivot column followed by a lane-parallel rank-1 update,
-// and the same wave then inverts the factor (lane c owns column c of L^-1),
-// so the panel solve of the blocked factorisation becomes an MFMA GEMM.
+// Small-block kernels of the triangular solves: the diagonal-block
+// triangular inverses, the in-place application of an inverted diagonal
+// block, and BLAS-2 helpers.
 #include "rmh_internal.hpp"
 
 namespace rmh {
@@ -31,57 +23,6 @@ __device__ __forceinline__ void invert_lower_lds(const double* L, double* X, int
         }
         if (c < i) X[i * kLdsLd + c] = unit ? -acc : -acc / L[i * kLdsLd + i];
     }
-}
-
-template <typename T>
-__global__ __launch_bounds__(64) void potrf_diag_kernel(T* A, int64_t rs, int64_t cs, int kb, T* W, int64_t ldw,
-                                                        int64_t* info, int64_t info_base) {
-    __shared__ double L[kDiagMax * kLdsLd];
-    __shared__ double X[kDiagMax * kLdsLd];
-    if (*info != 0) return;  // an earlier block already failed: LAPACK stops there
-    const int lane = threadIdx.x;
-
-    // load the lower triangle, lane = row
-    if (lane < kb)
-        for (int j = 0; j <= lane; ++j) L[lane * kLdsLd + j] = (double)A[lane * rs + j * cs];
-    __syncthreads();
-
-    int failed = -1;
-    for (int j = 0; j < kb; ++j) {
-        const double d = L[j * kLdsLd + j];
-        if (!(d > 0.0)) {  // also catches NaN
-            failed = j;
-            break;
-        }
-        const double s = sqrt(d);
-        double lij = 0.0;
-        if (lane > j && lane < kb) {
-            lij = L[lane * kLdsLd + j] / s;
-            L[lane * kLdsLd + j] = lij;
-        }
-        __syncthreads();
-        if (lane == 0) L[j * kLdsLd + j] = s;
-        // rank-1 update of the trailing lower triangle: row `lane`, cols j+1..lane
-        for (int c = j + 1; c < kb; ++c) {
-            const double lcj = L[c * kLdsLd + j];
-            if (lane >= c && lane < kb) L[lane * kLdsLd + c] = fma(-lij, lcj, L[lane * kLdsLd + c]);
-        }
-        __syncthreads();
-    }
-
-    // write back what was computed (all of it on success)
-    if (lane < kb)
-        for (int j = 0; j <= lane; ++j) A[lane * rs + j * cs] = (T)L[lane * kLdsLd + j];
-    if (failed >= 0) {
-        if (lane == 0) *info = info_base + failed + 1;
-        return;
-    }
-    __syncthreads();
-    invert_lower_lds(L, X, kb, false);
-    __syncthreads();
-    // W column-major, zero above the diagonal; lane = row for coalesced stores
-    if (lane < kb)
-        for (int c = 0; c < kb; ++c) W[lane + c * ldw] = (T)(lane >= c ? X[lane * kLdsLd + c] : 0.0);
 }
 
 // Invert the nb x nb diagonal blocks of op(A).  One wavefront per block.
@@ -120,6 +61,31 @@ __global__ __launch_bounds__(64) void trtri_blocks_kernel(const T* A, int64_t ld
             D[lane + c * (int64_t)nb] = (T)v;
         }
     }
+}
+
+// Inverses of the 64 x 64 diagonal blocks of the lower-triangular S
+// (S = A, or A^T when s_is_At), written into the block-diagonal strip D:
+// D holds inv(S_tt) of each nbd-row block t with element (i, j) of the
+// inverse at D[i + (j mod nbd)*ldd].  64-block b lands at rows [64b, 64b+64)
+// and local columns [(64b mod nbd), ...+64).  One wavefront per block.
+template <typename T>
+__global__ __launch_bounds__(64) void trtri_strip_kernel(const T* A, int64_t lda, int64_t n, bool s_is_At, bool unit,
+                                                         T* D, int64_t ldd, int64_t nbd) {
+    __shared__ double L[kDiagMax * kLdsLd];
+    __shared__ double X[kDiagMax * kLdsLd];
+    const int64_t r0 = (int64_t)blockIdx.x * kDiagMax;
+    const int kb = (int)min((int64_t)kDiagMax, n - r0);
+    const int lane = threadIdx.x;
+    if (lane < kb)
+        for (int j = 0; j <= lane; ++j)
+            L[lane * kLdsLd + j] =
+                (double)(s_is_At ? A[(r0 + j) + (r0 + lane) * lda] : A[(r0 + lane) + (r0 + j) * lda]);
+    __syncthreads();
+    invert_lower_lds(L, X, kb, unit);
+    __syncthreads();
+    T* Dblk = D + r0 + (r0 % nbd) * ldd;
+    if (lane < kb)
+        for (int c = 0; c < kb; ++c) Dblk[lane + c * ldd] = (T)(lane >= c ? X[lane * kLdsLd + c] : 0.0);
 }
 
 // B_blk <- D * B_blk (left) / B_blk * D (right), kb <= 64, one workgroup per
@@ -226,15 +192,6 @@ unsigned grid_for(int64_t n, int block) {
 }  // namespace
 
 template <typename T>
-rmh_status launch_potrf_diag(T* A, int64_t rs, int64_t cs, int kb, T* W, int64_t ldw, int64_t* info,
-                             int64_t info_base, hipStream_t stream) {
-    if (kb <= 0) return RMH_STATUS_SUCCESS;
-    if (kb > kDiagMax) return fail(RMH_STATUS_INVALID_VALUE, "potrf_diag: block larger than 64");
-    hipLaunchKernelGGL(potrf_diag_kernel<T>, dim3(1), dim3(64), 0, stream, A, rs, cs, kb, W, ldw, info, info_base);
-    return check_launch("potrf_diag_kernel");
-}
-
-template <typename T>
 rmh_status launch_trtri_blocks(const T* A, int64_t lda, int64_t n, int nb, bool upper, bool trans, bool unit,
                                T* Dinv, hipStream_t stream) {
     if (n <= 0) return RMH_STATUS_SUCCESS;
@@ -244,6 +201,17 @@ rmh_status launch_trtri_blocks(const T* A, int64_t lda, int64_t n, int nb, bool 
     hipLaunchKernelGGL(trtri_blocks_kernel<T>, dim3((unsigned)nblk), dim3(64), 0, stream, A, lda, n, nb, eff_upper,
                        trans, unit, Dinv);
     return check_launch("trtri_blocks_kernel");
+}
+
+template <typename T>
+rmh_status launch_trtri_strip(const T* A, int64_t lda, int64_t n, bool s_is_At, bool unit, T* D, int64_t ldd,
+                              int64_t nbd, hipStream_t stream) {
+    if (n <= 0) return RMH_STATUS_SUCCESS;
+    if (nbd % kDiagMax != 0) return fail(RMH_STATUS_INVALID_VALUE, "trtri_strip: block not a multiple of 64");
+    const int64_t nblk = (n + kDiagMax - 1) / kDiagMax;
+    hipLaunchKernelGGL(trtri_strip_kernel<T>, dim3((unsigned)nblk), dim3(64), 0, stream, A, lda, n, s_is_At, unit, D, ldd,
+                       nbd);
+    return check_launch("trtri_strip_kernel");
 }
 
 template <typename T>
@@ -295,12 +263,12 @@ rmh_status launch_fill_i64(int64_t* p, int64_t n, int64_t value, hipStream_t str
 }
 
 #define RMH_INST(T)                                                                                         \
-    template rmh_status launch_potrf_diag<T>(T*, int64_t, int64_t, int, T*, int64_t, int64_t*, int64_t,     \
-                                             hipStream_t);                                                 \
     template rmh_status launch_trtri_blocks<T>(const T*, int64_t, int64_t, int, bool, bool, bool, T*,       \
                                                hipStream_t);                                               \
     template rmh_status launch_tri_apply<T>(bool, const T*, int64_t, int, T*, int64_t, int64_t, int64_t, hipStream_t); \
     template rmh_status launch_scale<T>(T*, int64_t, int64_t, int64_t, T, hipStream_t);                    \
+    template rmh_status launch_trtri_strip<T>(const T*, int64_t, int64_t, bool, bool, T*, int64_t, int64_t,  \
+                                              hipStream_t);                                               \
     template rmh_status launch_gemv<T>(bool, int64_t, int64_t, T, const T*, int64_t, const T*, int64_t, T, \
                                        T*, int64_t, hipStream_t);                                          \
     template rmh_status launch_strided_copy<T>(const T*, int64_t, T*, int64_t, int64_t, hipStream_t);

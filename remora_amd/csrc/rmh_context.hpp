@@ -46,6 +46,10 @@ struct Context {
 template <typename T>
 rmh_status trsm_impl(Context* h, bool left, bool upper, bool trans, bool unit, int64_t m, int64_t n, T alpha,
                      const T* A, int64_t lda, T* B, int64_t ldb, hipStream_t s);
+bool trsm_blocked_applies(int64_t na, int64_t other);
+template <typename T>
+rmh_status trsm_blocked(Context* h, bool left, bool upper, bool trans, bool unit, int64_t m, int64_t n, T alpha,
+                        const T* A, int64_t lda, T* B, int64_t ldb, hipStream_t s);
 template <typename T>
 rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int64_t* d_info, hipStream_t s);
 template <typename T>

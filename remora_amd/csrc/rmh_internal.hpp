@@ -55,6 +55,15 @@ struct GemmDesc {
     bool transA = false, transB = false;
     Part part = Part::Full;
     const int64_t* abort_flag = nullptr;  // skip the launch's work when *abort_flag != 0
+    // strided batch of `batch` products: product b uses y = b % batch_ny,
+    // z = b / batch_ny and operand origins A + y*sAy + z*sAz (same for B, C)
+    int64_t batch = 1, batch_ny = 1;
+    int64_t sAy = 0, sAz = 0, sBy = 0, sBz = 0, sCy = 0, sCz = 0;
+    // split-K over `splits` chunks of K; partials go to split_ws
+    // (splits * M * N elements), then one reduction writes C
+    int splits = 1;
+    T* split_ws = nullptr;
+    int64_t k_chunk = 0;  // set by launch_gemm
 };
 
 // Launch C = alpha*op(A)*op(B) + beta*C (restricted to `part`).  `tile`
@@ -71,14 +80,13 @@ rmh_status launch_symm(const GemmDesc<T>& d, bool upper, hipStream_t stream);
 Tile pick_tile(int64_t M, int64_t N, Part part);
 
 // ------------------------------------------------- small LAPACK kernels --
-// Factor the kb x kb (kb <= 64) diagonal block whose (i,j) element is at
-// A[i*rs + j*cs] (lower triangle, i >= j), write L back, and write its
-// inverse to W (column-major, ldw, full kb x kb with zeros above the
-// diagonal).  On a non-positive pivot j: *info = info_base + j + 1.
-// Computes in FP64 whatever T is.
+// Factor the kb x kb (kb <= 64) diagonal block of a Cholesky panel and
+// solve the rows below it: V(i,c) = A[i*rs + c*cs], rows [0, rows), columns
+// [0, kb), V(0,0) on the diagonal.  On a non-positive pivot j:
+// *info = info_base + j + 1.  Computes in FP64 whatever T is.
 template <typename T>
-rmh_status launch_potrf_diag(T* A, int64_t rs, int64_t cs, int kb, T* W, int64_t ldw, int64_t* info,
-                             int64_t info_base, hipStream_t stream);
+rmh_status launch_potrf_panel(T* A, int64_t rs, int64_t cs, int64_t rows, int kb, int64_t* info, int64_t info_base,
+                              hipStream_t stream);
 
 // Invert every nb x nb diagonal block of the triangular matrix op(A)
 // (n x n).  eff_upper says whether op(A) is upper triangular.  Block b is
@@ -94,6 +102,13 @@ rmh_status launch_trtri_blocks(const T* A, int64_t lda, int64_t n, int nb, bool 
 template <typename T>
 rmh_status launch_tri_apply(bool left, const T* Dinv, int64_t ldd, int kb, T* B, int64_t ldb, int64_t r0,
                             int64_t other, hipStream_t stream);
+
+// Inverses of the 64-blocks of the lower-triangular S = (s_is_At ? A^T : A)
+// into a block-diagonal strip D (n x nbd, ld ldd): element (i, j) of the
+// inverse of an nbd-block lives at D[i + (j mod nbd)*ldd].
+template <typename T>
+rmh_status launch_trtri_strip(const T* A, int64_t lda, int64_t n, bool s_is_At, bool unit, T* D, int64_t ldd,
+                              int64_t nbd, hipStream_t stream);
 
 template <typename T>
 rmh_status launch_scale(T* B, int64_t ldb, int64_t rows, int64_t cols, T alpha, hipStream_t stream);

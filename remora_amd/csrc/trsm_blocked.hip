@@ -1,0 +1,190 @@
+// Blocked triangular solve for tall systems with few right-hand sides
+// (the two solves of potrs, the refinement corrections).
+//
+// A recursive solve splits into ever smaller GEMMs whose output is only
+// nrhs columns wide: they cannot fill 256 CUs, and hundreds of dependent
+// launches dominate.  Here the triangular matrix is cut into nbd-row blocks
+// (nbd = 512 for large n) and the inverses of its diagonal blocks are formed
+// first, hierarchically and in parallel: the 64-block inverses in one
+// launch, then for s = 64, 128, ... the 2s-block inverse from its halves,
+//     inv([P 0; Q R]) = [inv(P) 0; -inv(R) Q inv(P)  inv(R)],
+// as two strided-batched GEMMs per level over all 2s-blocks of the matrix.
+// The solve is then nbd-blocked substitution with two kernels per block:
+//     X_b = inv(T_bb) B_b               (split-K GEMM: kb x nrhs x kb)
+//     B_rest -= T_rest,b X_b            (tall GEMM: up to n x nrhs x nbd)
+// Using explicit inverses of diagonal blocks (as MAGMA's trsm does) costs
+// accuracy on the order of cond(T_bb) u; the Cholesky factors this serves
+// have cond(L_bb) <= sqrt(cond(A_bb)).
+#include <algorithm>
+
+#include "rmh_context.hpp"
+
+namespace rmh {
+namespace {
+
+template <typename T>
+struct SView {
+    const T* A;
+    int64_t lda;
+    bool s_is_At;  // S = A^T (upper storage) or S = A (lower storage)
+    // S[r0:, c0:] as a GEMM operand (origin pointer, transpose flag)
+    void block(int64_t r0, int64_t c0, const T** p, bool* tr) const {
+        if (!s_is_At) { *p = A + r0 + c0 * lda; *tr = false; }
+        else { *p = A + c0 + r0 * lda; *tr = true; }
+    }
+};
+
+int64_t choose_nbd(int64_t na) {
+    if (na > 4096) return 512;
+    if (na > 1024) return 256;
+    return 128;
+}
+
+// D (na x nbd strip, ld na) <- inverses of the nbd diagonal blocks of S.
+template <typename T>
+rmh_status build_block_inverses(const SView<T>& S, int64_t na, bool unit, int64_t nbd, T* D, T* W, hipStream_t s) {
+    RMH_TRY(launch_scale<T>(D, na, na, nbd, T(0), s));
+    RMH_TRY(launch_trtri_strip<T>(S.A, S.lda, na, S.s_is_At, unit, D, na, nbd, s));
+    for (int64_t h = 64; 2 * h <= nbd; h *= 2) {
+        const int64_t ny = nbd / (2 * h);
+        const int64_t full = na / (2 * h);  // pairs with both halves complete
+        if (full > 0) {
+            // W_p = Q_p inv(P_p)   (Q_p = S[r0+h : r0+2h, r0 : r0+h], r0 = 2h p)
+            GemmDesc<T> g1;
+            g1.M = h; g1.N = h; g1.K = h; g1.alpha = T(1); g1.beta = T(0);
+            S.block(h, 0, &g1.A, &g1.transA);
+            g1.lda = S.lda;
+            g1.B = D; g1.ldb = na; g1.transB = false;
+            g1.C = W; g1.ldc = h;
+            g1.batch = full; g1.batch_ny = ny;
+            g1.sAy = 2 * h * (1 + S.lda); g1.sAz = ny * g1.sAy;
+            g1.sBy = 2 * h * (1 + na);    g1.sBz = ny * 2 * h;
+            g1.sCy = h * h;               g1.sCz = ny * h * h;
+            RMH_TRY(launch_gemm<T>(g1, Tile::T64x64, s));
+            // X_p = -inv(R_p) W_p  into the strip below inv(P_p)
+            GemmDesc<T> g2;
+            g2.M = h; g2.N = h; g2.K = h; g2.alpha = T(-1); g2.beta = T(0);
+            g2.A = D + h + h * na; g2.lda = na; g2.transA = false;
+            g2.B = W; g2.ldb = h; g2.transB = false;
+            g2.C = D + h; g2.ldc = na;
+            g2.batch = full; g2.batch_ny = ny;
+            g2.sAy = 2 * h * (1 + na); g2.sAz = ny * 2 * h;
+            g2.sBy = h * h;            g2.sBz = ny * h * h;
+            g2.sCy = 2 * h * (1 + na); g2.sCz = ny * 2 * h;
+            RMH_TRY(launch_gemm<T>(g2, Tile::T64x64, s));
+        }
+        const int64_t r0 = full * 2 * h;
+        if (na - r0 > h) {  // the last pair has a short lower half
+            const int64_t sc = na - r0 - h, o = r0 % nbd;
+            GemmDesc<T> g1;
+            g1.M = sc; g1.N = h; g1.K = h; g1.alpha = T(1); g1.beta = T(0);
+            S.block(r0 + h, r0, &g1.A, &g1.transA);
+            g1.lda = S.lda;
+            g1.B = D + r0 + o * na; g1.ldb = na;
+            g1.C = W; g1.ldc = h;
+            RMH_TRY(launch_gemm<T>(g1, Tile::T64x64, s));
+            GemmDesc<T> g2;
+            g2.M = sc; g2.N = h; g2.K = sc; g2.alpha = T(-1); g2.beta = T(0);
+            g2.A = D + (r0 + h) + (o + h) * na; g2.lda = na;
+            g2.B = W; g2.ldb = h;
+            g2.C = D + (r0 + h) + o * na; g2.ldc = na;
+            RMH_TRY(launch_gemm<T>(g2, Tile::T64x64, s));
+        }
+    }
+    return RMH_STATUS_SUCCESS;
+}
+
+int split_count(int64_t M, int64_t N, int64_t K) {
+    const int64_t tiles = ((M + 63) / 64) * ((N + 63) / 64);
+    int64_t sp = (256 + tiles - 1) / tiles;
+    sp = std::min<int64_t>(sp, std::max<int64_t>(1, K / 32));
+    return (int)std::max<int64_t>(2, std::min<int64_t>(sp, 64));
+}
+
+}  // namespace
+
+bool trsm_blocked_applies(int64_t na, int64_t other) { return na >= 512 && other <= 1024 && other * 4 <= na; }
+
+template <typename T>
+rmh_status trsm_blocked(Context* h, bool left, bool upper, bool trans, bool unit, int64_t m, int64_t n, T alpha,
+                        const T* A, int64_t lda, T* B, int64_t ldb, hipStream_t s) {
+    const int64_t na = left ? m : n, other = left ? n : m;
+    const int64_t nbd = choose_nbd(na);
+    const SView<T> S{A, lda, upper};
+    const bool eff_upper = upper != trans;  // op(A) = S^T
+    // workspace: D strip | W (level products) | split-K partials
+    const size_t d_el = (size_t)na * nbd, w_el = (size_t)na * nbd / 2 + 64 * 64;
+    const int max_split = 64;
+    const size_t p_el = (size_t)max_split * nbd * other;
+    RMH_TRY(h->reserve((d_el + w_el + p_el) * sizeof(T) + 3 * 256));
+    T* D = static_cast<T*>(h->ws);
+    T* W = D + (d_el + 63) / 64 * 64;
+    T* P = W + (w_el + 63) / 64 * 64;
+
+    if (alpha != T(1)) RMH_TRY(launch_scale<T>(B, ldb, m, n, alpha, s));
+    if (alpha == T(0)) return RMH_STATUS_SUCCESS;
+    RMH_TRY(build_block_inverses<T>(S, na, unit, nbd, D, W, s));
+
+    // op(A)[r0:r1, c0:c1] as an operand: S block, or a transposed S block
+    auto opA = [&](int64_t r0, int64_t c0, const T** p, bool* tr) {
+        if (!eff_upper) S.block(r0, c0, p, tr);
+        else { S.block(c0, r0, p, tr); *tr = !*tr; }
+    };
+    const int64_t nblk = (na + nbd - 1) / nbd;
+    // left & lower or right & upper: front to back
+    const bool forward = left ? !eff_upper : eff_upper;
+    for (int64_t q = 0; q < nblk; ++q) {
+        const int64_t b = forward ? q : nblk - 1 - q;
+        const int64_t r0 = b * nbd, kb = std::min(nbd, na - r0);
+        const T* Db = D + r0;  // inv(S_bb), ld na; inv(op(A)_bb) = eff_upper ? Db^T : Db
+        GemmDesc<T> g;
+        g.alpha = T(1); g.beta = T(0);
+        if (left) {  // X_b = inv(op(A)_bb) B_b
+            g.M = kb; g.N = other; g.K = kb;
+            g.A = Db; g.lda = na; g.transA = eff_upper;
+            g.B = B + r0; g.ldb = ldb;
+            g.C = B + r0; g.ldc = ldb;
+        } else {     // X_b = B_b inv(op(A)_bb)
+            g.M = other; g.N = kb; g.K = kb;
+            g.A = B + r0 * ldb; g.lda = ldb;
+            g.B = Db; g.ldb = na; g.transB = eff_upper;
+            g.C = B + r0 * ldb; g.ldc = ldb;
+        }
+        // C aliases an input: the split-K path reads everything before its
+        // reduction writes C (a single m-tile cannot race either)
+        g.splits = split_count(g.M, g.N, g.K);
+        g.split_ws = P;
+        if ((int64_t)g.splits * g.M * g.N > (int64_t)p_el) return fail(RMH_STATUS_INVALID_VALUE, "trsm: split workspace");
+        RMH_TRY(launch_gemm<T>(g, Tile::T64x64, s));
+
+        // update the blocks not yet solved
+        int64_t u0, u1;  // index range of the rows (left) / columns (right) to update
+        if (forward) { u0 = r0 + kb; u1 = na; }
+        else { u0 = 0; u1 = r0; }
+        if (u1 <= u0) continue;
+        GemmDesc<T> u;
+        u.alpha = T(-1); u.beta = T(1);
+        if (left) {  // B[u0:u1] -= op(A)[u0:u1, r0:r0+kb] X_b
+            u.M = u1 - u0; u.N = other; u.K = kb;
+            opA(u0, r0, &u.A, &u.transA);
+            u.lda = lda;
+            u.B = B + r0; u.ldb = ldb;
+            u.C = B + u0; u.ldc = ldb;
+        } else {     // B[:, u0:u1] -= X_b op(A)[r0:r0+kb, u0:u1]
+            u.M = other; u.N = u1 - u0; u.K = kb;
+            u.A = B + r0 * ldb; u.lda = ldb;
+            opA(r0, u0, &u.B, &u.transB);
+            u.ldb = lda;
+            u.C = B + u0 * ldb; u.ldc = ldb;
+        }
+        RMH_TRY(launch_gemm<T>(u, pick_tile(u.M, u.N, Part::Full), s));
+    }
+    return RMH_STATUS_SUCCESS;
+}
+
+template rmh_status trsm_blocked<double>(Context*, bool, bool, bool, bool, int64_t, int64_t, double, const double*,
+                                         int64_t, double*, int64_t, hipStream_t);
+template rmh_status trsm_blocked<float>(Context*, bool, bool, bool, bool, int64_t, int64_t, float, const float*,
+                                        int64_t, float*, int64_t, hipStream_t);
+
+}  // namespace rmh

@@ -92,8 +92,8 @@ void syrk_test() {
         if (upper_case) kernels::syrk<true>(darg, dres, 2.0f);
         else kernels::syrk<false>(darg, dres, 2.0f);
         auto res = copy_to_cpu(dres);
-        for (std::size_t i = 0; i != dims; i += 7)
-            for (std::size_t j = 0; j != dims; j += 5) {
+        for (std::size_t i = 0; i < dims; i += 7)
+            for (std::size_t j = 0; j < dims; j += 5) {
                 bool touched = upper_case ? j >= i : j <= i;
                 if (!touched) {
                     CHECK_CLOSE_PCT(res(i, j), 3.0, 1e-10);

@@ -109,18 +109,24 @@ def test_dsyrk_reference_fixture(handle, upper, e_row_major):
 @pytest.mark.parametrize("upper", [False, True])
 @pytest.mark.parametrize("trans", [False, True])
 @pytest.mark.parametrize("nk", [(936, 1039), (1, 7), (4099, 256)])
-def test_dsyrk_random(handle, upper, trans, nk):
+@pytest.mark.parametrize("prec", ["d", "s"])
+def test_syrk_random(handle, upper, trans, nk, prec):
+    import torch
     from remora_amd import linalg as la
     n, k = nk
+    dt = torch.float64 if prec == "d" else torch.float32
     E = synth.normal(n + k, 0, n * k).reshape(n, k)
     C0 = synth.normal(n + k, 1, n * n).reshape(n, n)
-    dC = dev(C0, ld=n + 3)
-    la.syrk(handle, dev(E.T.copy() if trans else E), dC, alpha=-1.0, beta=0.5, upper=upper, trans=trans)
+    if prec == "s":
+        E = E.astype(np.float32).astype(np.float64)
+        C0 = C0.astype(np.float32).astype(np.float64)
+    dC = dev(C0, dtype=dt, ld=n + 3)
+    la.syrk(handle, dev(E.T.copy() if trans else E, dtype=dt), dC, alpha=-1.0, beta=0.5, upper=upper, trans=trans)
     got = host(dC)
     ref = 0.5 * C0 - E @ E.T
     mask = np.triu(np.ones((n, n), bool)) if upper else np.tril(np.ones((n, n), bool))
     scale = np.abs(E) @ np.abs(E.T) + 0.5 * np.abs(C0)
-    ok, worst = componentwise_ok(got[mask], ref[mask], scale[mask], TOL["d"])
+    ok, worst = componentwise_ok(got[mask], ref[mask], scale[mask], TOL[prec])
     assert ok, worst
     assert np.array_equal(got[~mask], C0[~mask])
 
