@@ -147,6 +147,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--nb", type=int, default=0, help="potrf panel width (0: library default)")
     ap.add_argument("--nbi", type=int, default=0, help="potrf diagonal block (0: library default)")
+    ap.add_argument("--no-lookahead", action="store_true", help="single-stream factorisation")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -159,9 +160,11 @@ def main():
     work = n ** 3 / 3 + 2 * n * n * nrhs
     log(f"[bench] inputs n={n} nrhs={nrhs}")
     h = capi.Handle(0)
+    from remora_amd import linalg as la
     if args.nb or args.nbi:
-        from remora_amd import linalg as la
         la.set_potrf_blocking(h, args.nb, args.nbi)
+    if args.no_lookahead:
+        la.set_potrf_lookahead(h, False)
     A, B = make_inputs(n, nrhs)
     torch.cuda.synchronize()
     log("[bench] fp64 potrf+potrs")

@@ -142,6 +142,10 @@ rmh_status rmh_dsposv(rmh_handle h, int upper, rmh_factor_precision prec, size_t
  * trailing update), nb_inner the diagonal block inside a panel (<= 64).
  * 0 restores the defaults.  Applies to later calls on this handle. */
 rmh_status rmh_set_potrf_blocking(rmh_handle h, size_t nb, size_t nb_inner);
+/* Look-ahead (on by default): rmh_?potrf factors panel k+1 on a second,
+ * handle-owned stream while the caller's stream applies the rest of panel
+ * k's trailing update; the call still completes in the caller's stream order. */
+rmh_status rmh_set_potrf_lookahead(rmh_handle h, int enable);
 
 /* ---------------------------------------------------------- profiling -- */
 /* Kernel classes: bit 0 "trailing_update", bit 1 "panel", bit 2 "diag",

@@ -61,6 +61,7 @@ SIGNATURES = {
     "rmh_spotrs": (ST, [P, I, S, S, P, S, P, S, P]),
     "rmh_dsposv": (ST, [P, I, I, S, S, P, S, P, S, P, S, D, I, C.POINTER(I), C.POINTER(D), P]),
     "rmh_set_potrf_blocking": (ST, [P, S, S]),
+    "rmh_set_potrf_lookahead": (ST, [P, I]),
     "rmh_profile_enable": (ST, [P, I]),
     "rmh_profile_reset": (ST, [P]),
     "rmh_profile_read": (ST, [P, C.c_char_p, C.POINTER(D), C.POINTER(D), C.POINTER(I64)]),

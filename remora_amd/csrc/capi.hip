@@ -72,6 +72,18 @@ rmh_status Context::reserve_slot(int slot, size_t bytes, void** ptr) {
     return RMH_STATUS_SUCCESS;
 }
 
+rmh_status Context::side_stream(hipStream_t* st, hipEvent_t* ready, hipEvent_t* done) {
+    if (side == nullptr) {
+        RMH_TRY_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));  // default priority
+        RMH_TRY_HIP(hipEventCreateWithFlags(&ev_a, hipEventDisableTiming));
+        RMH_TRY_HIP(hipEventCreateWithFlags(&ev_b, hipEventDisableTiming));
+    }
+    *st = side;
+    *ready = ev_a;
+    *done = ev_b;
+    return RMH_STATUS_SUCCESS;
+}
+
 rmh_status Context::prof_begin(hipStream_t s, int cls, double flops, int* slot) {
     *slot = -1;
     if (((prof_mask >> cls) & 1) == 0) return RMH_STATUS_SUCCESS;
@@ -103,6 +115,12 @@ void Context::prof_clear() {
 Context::~Context() {
     prof_clear();
     (void)hipSetDevice(device);
+    if (side != nullptr) {
+        (void)hipStreamSynchronize(side);
+        (void)hipStreamDestroy(side);
+        (void)hipEventDestroy(ev_a);
+        (void)hipEventDestroy(ev_b);
+    }
     if (ws != nullptr) (void)hipFree(ws);
     for (int i = 0; i < WS_COUNT; ++i)
         if (aux[i] != nullptr) (void)hipFree(aux[i]);
@@ -211,71 +229,109 @@ rmh_status trsm_impl(Context* h, bool left, bool upper, bool trans, bool unit, i
 //
 // For each panel of width nb:
 //   for each diagonal block of width nbi inside the panel:
+//     inner  -- update of this block column by the panel's earlier columns
 //     panel  -- factor V_jj in LDS and solve the rows below against it
 //               (one fused launch, potrf_panel_kernel)
-//     inner  -- rank-nbi update of the rest of the panel's columns
-//   trailing -- V22 -= P P^T with K = nb (MFMA SYRK on the lower trapezoid)
-// Every launch reads the device info word first and does nothing once a
-// pivot has failed.
+//   trailing -- V22 -= P P^T with K = nb (MFMA SYRK on the lower trapezoid),
+//               the next panel's columns first so that panel can be factored
+//               on a second stream meanwhile (look-ahead)
+// A failed pivot records LAPACK's info (the first failure wins); the
+// remaining launches still run, on values that no longer matter.
 template <typename T>
 rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int64_t* d_info, hipStream_t s) {
     const int64_t nb = (int64_t)h->nb, nbi = (int64_t)h->nbi;
     RMH_TRY(h->reserve(256));
-    int64_t* info = d_info != nullptr ? d_info : static_cast<int64_t*>(h->ws);
+    // scratch words: [0] info when the caller passes none, [1] panel reader counter
+    int64_t* scratch = static_cast<int64_t*>(h->ws);
+    int64_t* info = d_info != nullptr ? d_info : scratch;
+    unsigned* readers = reinterpret_cast<unsigned*>(scratch + 1);
     RMH_TRY(launch_fill_i64(info, 1, 0, s));
+    RMH_TRY(launch_fill_i64(scratch + 1, 1, 0, s));
     if (n == 0) return RMH_STATUS_SUCCESS;
     const int64_t rs = upper ? lda : 1, cs = upper ? 1 : lda;
     auto V = [&](int64_t i, int64_t j) { return A + i * rs + j * cs; };
 
-    for (int64_t k0 = 0; k0 < n; k0 += nb) {
-        const int64_t kw = std::min(nb, n - k0);
+    // Factor the panel V[k0:, k0:k0+kw] left-looking: before the fused
+    // panel kernel factors block column [j0, j0+jb), one GEMM applies the
+    // updates of the panel's earlier columns [k0, j0) to it, so each block
+    // column is read and written once.
+    auto factor_panel = [&](int64_t k0, int64_t kw, hipStream_t st) -> rmh_status {
         for (int64_t j0 = k0; j0 < k0 + kw; j0 += nbi) {
             const int64_t jb = std::min(nbi, k0 + kw - j0);
-            const int64_t below = n - j0 - jb;
+            const int64_t rows = n - j0;
             int slot;
-            RMH_TRY(h->prof_begin(s, PROF_PANEL, (double)jb * jb * jb / 3.0 + (double)below * jb * jb, &slot));
-            RMH_TRY(launch_potrf_panel<T>(V(j0, j0), rs, cs, n - j0, (int)jb, info, j0, s));
-            RMH_TRY(h->prof_end(s, slot));
-            const int64_t cw = k0 + kw - (j0 + jb);
-            if (cw > 0) {
-                // inner update of the remaining panel columns
+            if (j0 > k0) {
+                const int64_t K = j0 - k0;
                 GemmDesc<T> g;
-                g.alpha = T(-1); g.beta = T(1); g.abort_flag = info; g.K = jb;
+                g.alpha = T(-1); g.beta = T(1); g.K = K;
                 if (!upper) {
-                    g.M = below; g.N = cw; g.part = Part::Lower;
-                    g.A = A + (j0 + jb) + j0 * lda; g.lda = lda; g.transA = false;
+                    g.M = rows; g.N = jb; g.part = Part::Lower;
+                    g.A = A + j0 + k0 * lda; g.lda = lda; g.transA = false;
                     g.B = g.A; g.ldb = lda; g.transB = true;
-                    g.C = A + (j0 + jb) + (j0 + jb) * lda; g.ldc = lda;
                 } else {
-                    g.M = cw; g.N = below; g.part = Part::Upper;
-                    g.A = A + j0 + (j0 + jb) * lda; g.lda = lda; g.transA = true;
+                    g.M = jb; g.N = rows; g.part = Part::Upper;
+                    g.A = A + k0 + j0 * lda; g.lda = lda; g.transA = true;
                     g.B = g.A; g.ldb = lda; g.transB = false;
-                    g.C = A + (j0 + jb) + (j0 + jb) * lda; g.ldc = lda;
                 }
-                RMH_TRY(h->prof_begin(s, PROF_INNER, 2.0 * jb * ((double)cw * (cw + 1) / 2.0 + (double)(below - cw) * cw), &slot));
-                RMH_TRY(launch_gemm<T>(g, Tile::T64x64, s));
-                RMH_TRY(h->prof_end(s, slot));
+                g.C = A + j0 + j0 * lda; g.ldc = lda;
+                RMH_TRY(h->prof_begin(st, PROF_INNER, 2.0 * K * ((double)jb * (jb + 1) / 2.0 + (double)(rows - jb) * jb), &slot));
+                RMH_TRY(launch_gemm<T>(g, Tile::T64x64, st));
+                RMH_TRY(h->prof_end(st, slot));
             }
+            RMH_TRY(h->prof_begin(st, PROF_PANEL, (double)jb * jb * jb / 3.0 + (double)(rows - jb) * jb * jb, &slot));
+            RMH_TRY(launch_potrf_panel<T>(V(j0, j0), rs, cs, rows, (int)jb, info, j0, readers, st));
+            RMH_TRY(h->prof_end(st, slot));
         }
-        const int64_t rt = n - k0 - kw;
-        if (rt > 0) {
-            GemmDesc<T> g;
-            g.alpha = T(-1); g.beta = T(1); g.abort_flag = info;
-            g.M = rt; g.N = rt; g.K = kw;
-            if (!upper) {
-                g.part = Part::Lower;
-                g.A = A + (k0 + kw) + k0 * lda; g.lda = lda; g.transA = false;
-                g.B = g.A; g.ldb = lda; g.transB = true;
-            } else {
-                g.part = Part::Upper;
-                g.A = A + k0 + (k0 + kw) * lda; g.lda = lda; g.transA = true;
-                g.B = g.A; g.ldb = lda; g.transB = false;
-            }
-            g.C = A + (k0 + kw) + (k0 + kw) * lda; g.ldc = lda;
-            int slot;
-            RMH_TRY(h->prof_begin(s, PROF_TRAILING, (double)rt * (rt + 1) * kw, &slot));
-            RMH_TRY(launch_gemm<T>(g, pick_tile(rt, rt, g.part), s));
-            RMH_TRY(h->prof_end(s, slot));
+        return RMH_STATUS_SUCCESS;
+    };
+    // V[c0:, c0:c1] -= P P[c0:c1]^T on the lower trapezoid, P = V[c0:, k0:k0+kw]
+    auto trailing = [&](int64_t k0, int64_t kw, int64_t c0, int64_t c1, hipStream_t st) -> rmh_status {
+        const int64_t rows = n - c0, cols = c1 - c0;
+        if (rows <= 0 || cols <= 0) return RMH_STATUS_SUCCESS;
+        GemmDesc<T> g;
+        g.alpha = T(-1); g.beta = T(1); g.K = kw;
+        if (!upper) {
+            g.M = rows; g.N = cols; g.part = Part::Lower;
+            g.A = A + c0 + k0 * lda; g.lda = lda; g.transA = false;
+            g.B = g.A; g.ldb = lda; g.transB = true;
+        } else {
+            g.M = cols; g.N = rows; g.part = Part::Upper;
+            g.A = A + k0 + c0 * lda; g.lda = lda; g.transA = true;
+            g.B = g.A; g.ldb = lda; g.transB = false;
+        }
+        g.C = A + c0 + c0 * lda; g.ldc = lda;
+        const double flops = 2.0 * kw * ((double)cols * (cols + 1) / 2.0 + (double)(rows - cols) * cols);
+        int slot;
+        RMH_TRY(h->prof_begin(st, PROF_TRAILING, flops, &slot));
+        RMH_TRY(launch_gemm<T>(g, pick_tile(g.M, g.N, g.part), st));
+        return h->prof_end(st, slot);
+    };
+
+    // Look-ahead: while the caller's stream s applies panel k's update to
+    // the columns right of panel k+1, the side stream factors panel k+1
+    // (whose columns s updated first).  Both read panel k only; the column
+    // ranges they write are disjoint.
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_ready = nullptr, ev_done = nullptr;
+    const bool lookahead = h->lookahead && n > 2 * nb;
+    if (lookahead) RMH_TRY(h->side_stream(&s2, &ev_ready, &ev_done));
+    RMH_TRY(factor_panel(0, std::min(nb, n), s));
+    for (int64_t k0 = 0; k0 < n; k0 += nb) {
+        const int64_t kw = std::min(nb, n - k0);
+        const int64_t k1 = k0 + kw;  // next panel
+        if (k1 >= n) break;
+        const int64_t kw1 = std::min(nb, n - k1);
+        RMH_TRY(trailing(k0, kw, k1, k1 + kw1, s));  // next panel's columns first
+        if (lookahead) {
+            RMH_TRY_HIP(hipEventRecord(ev_ready, s));
+            RMH_TRY_HIP(hipStreamWaitEvent(s2, ev_ready, 0));
+            RMH_TRY(factor_panel(k1, kw1, s2));
+            RMH_TRY_HIP(hipEventRecord(ev_done, s2));
+            RMH_TRY(trailing(k0, kw, k1 + kw1, n, s));
+            RMH_TRY_HIP(hipStreamWaitEvent(s, ev_done, 0));
+        } else {
+            RMH_TRY(trailing(k0, kw, k1 + kw1, n, s));
+            RMH_TRY(factor_panel(k1, kw1, s));
         }
     }
     return RMH_STATUS_SUCCESS;
@@ -551,6 +607,12 @@ rmh_status rmh_dsposv(rmh_handle h, int upper, rmh_factor_precision prec, size_t
     RMH_REQUIRE(A != nullptr && B != nullptr && X != nullptr, "dsposv: NULL pointer");
     return dsposv_impl(ctx(h), upper != 0, prec, (int64_t)n, (int64_t)nrhs, A, (int64_t)lda, B, (int64_t)ldb, X,
                        (int64_t)ldx, tol, max_iter, iters, berr, hs(stream));
+}
+
+rmh_status rmh_set_potrf_lookahead(rmh_handle h, int enable) {
+    if (h == nullptr) return fail(RMH_STATUS_INVALID_HANDLE, "NULL handle");
+    ctx(h)->lookahead = enable != 0;
+    return RMH_STATUS_SUCCESS;
 }
 
 rmh_status rmh_set_potrf_blocking(rmh_handle h, size_t nb, size_t nb_inner) {

@@ -69,7 +69,7 @@ struct KParams {
     int64_t ldb;
     T* C;
     int64_t ldc;
-    int64_t tiles_m, tiles_n;
+    int64_t tiles_m, tiles_n, ntiles;
     const int64_t* abort_flag;
     // strided batch: blockIdx.z = b -> (y = b % batch_ny, z = b / batch_ny),
     // operand origins advance by y*s?y + z*s?z elements
@@ -306,17 +306,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     const T alpha = p.alpha, beta = p.beta;
     T* cbase = p.C + m0 + n0 * p.ldc;
     const int ldc32 = (int)p.ldc;
-    // For each accumulator row block: first load every C value it needs,
-    // then store, so the loads are in flight together (a store between two
-    // loads would otherwise order them).
+    // For each 16x16 accumulator: first load the C values it needs, then
+    // store (a store between two loads would otherwise order them).
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
         const int mm = wm * WM + i * 16 + lr;
         const bool row_ok = m0 + mm < M;
-        T cold[FN][4];
-        bool ok[FN][4];
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
+            T cold[4];
+            bool ok[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int nn = wn * WN + j * 16 + (MF::kRowInterleaved ? 4 * r + lk : 4 * lk + r);
@@ -324,18 +323,15 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
                 bool o = row_ok && gn < N;
                 if (PART == 1) o = o && (gm >= gn);
                 if (PART == 2) o = o && (gm <= gn);
-                ok[j][r] = o;
-                cold[j][r] = (o && beta != T(0)) ? cbase[mm + nn * ldc32] : T(0);
+                ok[r] = o;
+                cold[r] = (o && beta != T(0)) ? cbase[mm + nn * ldc32] : T(0);
             }
-        }
-#pragma unroll
-        for (int j = 0; j < FN; ++j) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                if (!ok[j][r]) continue;
+                if (!ok[r]) continue;
                 const int nn = wn * WN + j * 16 + (MF::kRowInterleaved ? 4 * r + lk : 4 * lk + r);
                 const T v = alpha * acc[i][j][r];
-                cbase[mm + nn * ldc32] = (beta == T(0)) ? v : v + beta * cold[j][r];
+                cbase[mm + nn * ldc32] = (beta == T(0)) ? v : v + beta * cold[r];
             }
         }
     }
@@ -380,6 +376,7 @@ rmh_status launch_cfg(const GemmDesc<T>& d, hipStream_t stream) {
     }
     if (ntiles <= 0) return RMH_STATUS_SUCCESS;
     if (ntiles > 0x7fffffffLL) return fail(RMH_STATUS_INVALID_VALUE, "gemm: problem too large for one launch");
+    p.ntiles = ntiles;
     dim3 grid((unsigned)ntiles, (unsigned)std::max(1, d.splits), (unsigned)std::max<int64_t>(1, d.batch)),
         block(kThreads);
     if (!d.transA && !d.transB)
@@ -426,6 +423,7 @@ rmh_status launch_symm_cfg(const GemmDesc<T>& d, hipStream_t stream) {
     p.split_ws = nullptr;
     const int64_t ntiles = p.tiles_m * p.tiles_n;
     if (ntiles <= 0) return RMH_STATUS_SUCCESS;
+    p.ntiles = ntiles;
     if (d.transB)
         hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, false, true, 0, SYM>), dim3((unsigned)ntiles), dim3(kThreads), 0,
                            stream, p);

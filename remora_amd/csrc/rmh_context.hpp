@@ -23,7 +23,7 @@ enum WsSlot { WS_MAIN = 0, WS_MIXED = 1, WS_FACTOR = 2, WS_BATCH = 3, WS_COUNT =
 
 struct Context {
     static constexpr size_t kDefaultNb = 256;
-    static constexpr size_t kDefaultNbi = 64;
+    static constexpr size_t kDefaultNbi = 32;
 
     int device = 0;
     void* ws = nullptr;  // WS_MAIN (potrf / trsm scratch)
@@ -32,11 +32,15 @@ struct Context {
     size_t aux_bytes[WS_COUNT] = {0, 0, 0, 0};
     size_t nb = kDefaultNb, nbi = kDefaultNbi;
     int prof_mask = 0;  // bit i: record launches of ProfClass i
+    bool lookahead = true;
+    hipStream_t side = nullptr;  // look-ahead stream of potrf (default priority)
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
     std::vector<ProfRec> recs;
 
     rmh_status activate();
     rmh_status reserve(size_t bytes);
     rmh_status reserve_slot(int slot, size_t bytes, void** ptr);
+    rmh_status side_stream(hipStream_t* st, hipEvent_t* ready, hipEvent_t* done);
     rmh_status prof_begin(hipStream_t s, int cls, double flops, int* slot);
     rmh_status prof_end(hipStream_t s, int slot);
     void prof_clear();

@@ -83,10 +83,11 @@ Tile pick_tile(int64_t M, int64_t N, Part part);
 // Factor the kb x kb (kb <= 64) diagonal block of a Cholesky panel and
 // solve the rows below it: V(i,c) = A[i*rs + c*cs], rows [0, rows), columns
 // [0, kb), V(0,0) on the diagonal.  On a non-positive pivot j:
-// *info = info_base + j + 1.  Computes in FP64 whatever T is.
+// *info = info_base + j + 1.  Computes in FP64 whatever T is.  *readers is
+// a zero counter in device memory; the kernel leaves it zero again.
 template <typename T>
 rmh_status launch_potrf_panel(T* A, int64_t rs, int64_t cs, int64_t rows, int kb, int64_t* info, int64_t info_base,
-                              hipStream_t stream);
+                              unsigned* readers, hipStream_t stream);
 
 // Invert every nb x nb diagonal block of the triangular matrix op(A)
 // (n x n).  eff_upper says whether op(A) is upper triangular.  Block b is

@@ -130,6 +130,10 @@ def set_potrf_blocking(h, nb=0, nb_inner=0):
     capi.check(capi.load().rmh_set_potrf_blocking(h.value, nb, nb_inner))
 
 
+def set_potrf_lookahead(h, enable=True):
+    capi.check(capi.load().rmh_set_potrf_lookahead(h.value, int(enable)))
+
+
 class Profile:
     """Launch profiler of a handle (HIP events around each launch class)."""
 

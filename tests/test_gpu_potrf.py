@@ -167,3 +167,42 @@ def test_potrf_potrs_backward_error(handle, upper, n, nrhs):
     if n <= 2049:
         Xref, _, _, _ = orc.posv(A, B)
         np.testing.assert_allclose(X, Xref, rtol=1e-10, atol=1e-12 * np.max(np.abs(Xref)))
+
+
+@pytest.mark.parametrize("upper", [False, True])
+def test_potrf_lookahead_is_bitwise_identical(handle, upper):
+    """Look-ahead only reorders independent launches across two streams: the
+    factor must be bit-identical to the single-stream schedule."""
+    from remora_amd import linalg as la
+    n = 1700
+    A = synth.gram_spd(n, seed=11)
+    outs = []
+    for on in (True, False):
+        la.set_potrf_lookahead(handle, on)
+        try:
+            dA = dev(A)
+            info = la.potrf(handle, dA, upper=upper)
+            outs.append(host(dA))
+            assert int(info.item()) == 0
+        finally:
+            la.set_potrf_lookahead(handle, True)
+    assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("nbi", [16, 32, 64])
+def test_potrf_large_lookahead_factor_error(handle, nbi):
+    """Large n with look-ahead: panel kernels run beside the trailing update,
+    so late workgroups of a panel launch are common -- the factor must stay
+    exact to n*u (guards the diagonal-block write-back ordering)."""
+    from remora_amd import linalg as la
+    n = 6000
+    A = synth.gram_spd(n, seed=21)
+    la.set_potrf_blocking(handle, 256, nbi)
+    try:
+        dA = dev(A)
+        info = la.potrf(handle, dA)
+        L = np.tril(host(dA))
+    finally:
+        la.set_potrf_blocking(handle, 0, 0)
+    assert int(info.item()) == 0
+    assert synth.factor_error(A, L) < 4 * n * np.finfo(float).eps
