@@ -1,19 +1,31 @@
-"""Benchmark: potrf+solve TFLOP/s at n=16384 with 64 right-hand sides.
+"""Benchmark: potrf+solve TFLOP/s at n=16384, FP64 vs FP32-MFMA + FP64 refinement.
 
 A step is Remora's solve(A, B, symm_pos_def(), left()) on device-resident
-inputs: copy A into the factor buffer and B into X (the copies
-cholesky_decomposition and matrix_matrix_solve make), factor, and solve.
-Work per step is the algorithmic n^3/3 + 2 n^2 nrhs flops.
+inputs with 64 right-hand sides:
+  fp64   copy A into the factor buffer and B into X (the copies
+         cholesky_decomposition and matrix_matrix_solve make), dpotrf, dpotrs;
+  mixed  rmh_dsposv: FP32 copy of A, Cholesky with FP32-MFMA trailing updates
+         and FP64 diagonal blocks, FP32 solve, then FP64 residual + FP32
+         correction until the backward error is <= 1e-14 (every iteration is
+         inside the timed step).
+Work per step is the algorithmic n^3/3 + 2 n^2 nrhs flops for both.
 
-Inputs: A = G G^T / n + 0.1 I with G n x n N(0,1) (seeded torch generator on
-the GPU), B n x 64 U[-1,1].  Every result carries ||A - L L^T||_F/||A||_F
-(estimated on sampled columns) and the backward error
-max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf).
+Workloads (both at n=16384, 64 RHS, B U[-1,1]):
+  rbf   A_ij = exp(-|x_i - x_j|^2 / (2 * 0.3^2)) + 1e-2 delta_ij, x_i in
+        [0,1]^8 (project RNG, seed 1): the mixed-precision comparison, both
+        paths on the same matrix;
+  gram  A = G G^T / n + 0.1 I with G N(0,1) (seeded torch generator): the
+        plain FP64 drop-in case.
+`value` is the mixed path's throughput on rbf at backward error <= 1e-14;
+the FP64 throughputs on rbf and gram and the ratio stand beside it.  Every
+result carries ||A - L L^T||_F/||A||_F (sampled columns) and the backward
+error max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf).
 
-The roofline object is measured live: HIP events around every launch of the
-trailing-update kernel (the dominant one) during the timed steps.  The CPU
-baseline is the oracle (a restatement of Remora's default CPU kernels) on one
-core, on a bounded sample: n=4096 with 64 RHS.
+The roofline objects are measured live: HIP events recorded by the library
+on the launching stream around every trailing-update launch (the dominant
+kernel) during the timed steps.  The CPU baseline is the oracle (a
+restatement of Remora's default CPU kernels) on one core, on a bounded
+sample: potrf + potrs at n=8192 with 64 RHS (about 15 s of CPU time).
 """
 from __future__ import annotations
 
@@ -26,15 +38,15 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP64 = 78.6e12
-PEAK_FP32 = 157.3e12
+PEAK = {"f64": 78.6e12, "f32": 157.3e12}
+TOL = 1e-14
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_inputs(n, nrhs, seed=0):
+def make_gram(n, nrhs, seed=0):
     import torch
     from remora_amd import linalg as la
     g = torch.Generator(device="cuda")
@@ -51,9 +63,24 @@ def make_inputs(n, nrhs, seed=0):
     return A, B
 
 
+def make_rbf(n, nrhs, d=8, lengthscale=0.3, jitter=1e-2, seed=1):
+    import torch
+    from remora_amd import linalg as la
+    from remora_amd import synth
+    x = torch.from_numpy(synth.points(n, d, seed)).cuda()
+    sq = (x * x).sum(dim=1)
+    d2 = (sq[:, None] + sq[None, :] - 2.0 * (x @ x.t())).clamp_min_(0.0)
+    A = torch.exp(d2.mul_(-1.0 / (2.0 * lengthscale * lengthscale)))
+    del d2
+    A = 0.5 * (A + A.t())
+    A.diagonal().add_(jitter)
+    A = la.colmajor(A)
+    B = la.colmajor(torch.from_numpy(synth.rhs(n, nrhs, seed=seed)).cuda())
+    return A, B
+
+
 def backward_error(h, A, X, B):
     """max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf) on the GPU."""
-    import torch
     from remora_amd import linalg as la
     R = la.colmajor(B.clone())
     la.gemm(h, A, X, R, alpha=-1.0, beta=1.0)
@@ -63,33 +90,23 @@ def backward_error(h, A, X, B):
     return float((num / den).max().item())
 
 
-def factor_error_sampled(h, A, L, cols=64, seed=1):
+def factor_error_sampled(A, L, cols=64, seed=1):
     """||A - L L^T||_F / ||A||_F estimated on sampled columns of A."""
     import torch
     n = A.shape[0]
     g = torch.Generator(device="cpu")
     g.manual_seed(seed)
     idx = torch.randperm(n, generator=g)[:cols].cuda()
-    Lt = torch.tril(L)
+    Lt = torch.tril(L.to(torch.float64))
     Ac = A[:, idx]
     Rc = Ac - Lt @ Lt[idx, :].t()
     return float((Rc.norm() / Ac.norm()).item())
 
 
-def run_fp64(h, A, B, steps, warmup, profile=True):
+def timed(step, steps, warmup, h, profile):
+    """Run warm-up steps, then time `steps` steps with events; returns per-step ms and the trailing profile."""
     import torch
     from remora_amd import linalg as la
-    n, nrhs = B.shape
-    L = la.empty_colmajor(n, n)
-    X = la.empty_colmajor(n, nrhs)
-    info = torch.zeros(1, dtype=torch.int64, device="cuda")
-
-    def step():
-        L.copy_(A)
-        X.copy_(B)
-        la.potrf(h, L, info=info)
-        la.potrs(h, L, X)
-
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
@@ -111,13 +128,57 @@ def run_fp64(h, A, B, steps, warmup, profile=True):
     if profile:
         prof.disable()
         tr = prof.read("trailing_update")
+    return per, wall, tr
+
+
+def run_fp64(h, A, B, steps, warmup, profile=True):
+    import torch
+    from remora_amd import linalg as la
+    n, nrhs = B.shape
+    L = la.empty_colmajor(n, n)
+    X = la.empty_colmajor(n, nrhs)
+    info = torch.zeros(1, dtype=torch.int64, device="cuda")
+
+    def step():
+        L.copy_(A)
+        X.copy_(B)
+        la.potrf(h, L, info=info)
+        la.potrs(h, L, X)
+
+    per, wall, tr = timed(step, steps, warmup, h, profile)
     assert int(info.item()) == 0, f"potrf info {int(info.item())}"
     return {"wall_s": wall, "step_ms": per, "trailing": tr, "L": L, "X": X}
 
 
-def cpu_baseline(n=4096, nrhs=64, reps=3):
+def run_mixed(h, A, B, steps, warmup, prec="fp32", profile=True):
+    from remora_amd import linalg as la
+    n, nrhs = B.shape
+    X = la.empty_colmajor(n, nrhs)
+    hist = []
+
+    def step():
+        hist.append(la.dsposv(h, A, B, X, prec=prec, tol=TOL, max_iter=30))
+
+    per, wall, tr = timed(step, steps, warmup, h, profile)
+    return {"wall_s": wall, "step_ms": per, "trailing": tr, "X": X, "iters": [i for i, _ in hist[warmup:]],
+            "berr": [b for _, b in hist[warmup:]]}
+
+
+def median(v):
+    s = sorted(v)
+    return s[len(s) // 2]
+
+
+def roofline(tr, dtype, kernel):
+    achieved = tr["flops"] / (tr["ms"] * 1e-3) if tr and tr["ms"] > 0 else None
+    return {"bound": "mfma", "achieved": achieved / 1e12 if achieved else None, "peak": PEAK[dtype] / 1e12,
+            "unit": "TFLOP/s", "frac": achieved / PEAK[dtype] if achieved else None, "traffic": None,
+            "kernel": kernel, "launches": tr["launches"] if tr else None,
+            "avg_launch_us": tr["ms"] * 1e3 / tr["launches"] if tr and tr["launches"] else None}
+
+
+def cpu_baseline(n=8192, nrhs=64, reps=3):
     """Oracle (Remora default CPU kernels restated) on one core."""
-    import numpy as np
     from oracle import oracle as orc
     from remora_amd import synth
     A = synth.gram_spd(n, seed=0)
@@ -133,7 +194,8 @@ def cpu_baseline(n=4096, nrhs=64, reps=3):
         eta = synth.backward_error(A, X, B)
     work = n ** 3 / 3 + 2 * n * n * nrhs
     return {"value": work / best / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
-            "sample": f"oracle posv n={n} nrhs={nrhs}, min of {reps} runs, 1 thread; backward error {eta:.2e}",
+            "sample": f"oracle potrf+potrs n={n} nrhs={nrhs} (Gram SPD), min of {reps} runs, 1 thread; "
+                      f"backward error {eta:.2e}; host has {os.cpu_count()} logical CPUs",
             "seconds": best}
 
 
@@ -145,6 +207,8 @@ def main():
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-gram", action="store_true", help="skip the FP64 Gram-matrix leg")
+    ap.add_argument("--only-fp64", action="store_true", help="FP64 legs only (no mixed leg)")
     ap.add_argument("--nb", type=int, default=0, help="potrf panel width (0: library default)")
     ap.add_argument("--nbi", type=int, default=0, help="potrf diagonal block (0: library default)")
     ap.add_argument("--no-lookahead", action="store_true", help="single-stream factorisation")
@@ -156,26 +220,61 @@ def main():
 
     import torch
     from remora_amd import capi
+    from remora_amd import linalg as la
     n, nrhs = args.n, args.nrhs
     work = n ** 3 / 3 + 2 * n * n * nrhs
-    log(f"[bench] inputs n={n} nrhs={nrhs}")
     h = capi.Handle(0)
-    from remora_amd import linalg as la
     if args.nb or args.nbi:
         la.set_potrf_blocking(h, args.nb, args.nbi)
     if args.no_lookahead:
         la.set_potrf_lookahead(h, False)
-    A, B = make_inputs(n, nrhs)
+    paths, acc = {}, {}
+
+    log(f"[bench] rbf inputs n={n} nrhs={nrhs}")
+    A, B = make_rbf(n, nrhs)
     torch.cuda.synchronize()
-    log("[bench] fp64 potrf+potrs")
-    r64 = run_fp64(h, A, B, args.steps, args.warmup)
-    med = sorted(r64["step_ms"])[len(r64["step_ms"]) // 2]
-    eta64 = backward_error(h, A, r64["X"], B)
-    ferr64 = factor_error_sampled(h, A, r64["L"])
-    tflops64 = work / (med * 1e-3) / 1e12
-    log(f"[bench] fp64: median {med:.3f} ms  {tflops64:.2f} TFLOP/s  eta {eta64:.2e}  ferr {ferr64:.2e}")
-    tr = r64["trailing"]
-    achieved = tr["flops"] / (tr["ms"] * 1e-3) if tr and tr["ms"] > 0 else None
+    log("[bench] rbf fp64 potrf+potrs")
+    r = run_fp64(h, A, B, args.steps, args.warmup)
+    med64 = median(r["step_ms"])
+    eta = backward_error(h, A, r["X"], B)
+    ferr = factor_error_sampled(A, r["L"])
+    paths["rbf_fp64"] = {"tflops": work / (med64 * 1e-3) / 1e12, "ms": med64, "step_ms": r["step_ms"]}
+    acc["rbf_fp64"] = {"backward_error": eta, "factor_error_sampled": ferr}
+    roof64 = roofline(r["trailing"], "f64", "gemm_kernel<double,128,128> trailing SYRK update (lower, K=256)")
+    log(f"[bench] rbf fp64: median {med64:.3f} ms  {paths['rbf_fp64']['tflops']:.2f} TFLOP/s  eta {eta:.2e}  "
+        f"ferr {ferr:.2e}")
+    del r
+    roof32 = None
+    if not args.only_fp64:
+        log("[bench] rbf mixed fp32 dsposv")
+        r = run_mixed(h, A, B, args.steps, args.warmup, "fp32")
+        medmx = median(r["step_ms"])
+        eta = backward_error(h, A, r["X"], B)
+        paths["rbf_mixed_fp32"] = {"tflops": work / (medmx * 1e-3) / 1e12, "ms": medmx, "step_ms": r["step_ms"],
+                                   "iters": r["iters"]}
+        acc["rbf_mixed_fp32"] = {"backward_error": eta, "reported_berr": r["berr"]}
+        roof32 = roofline(r["trailing"], "f32", "gemm_kernel<float,128,128> trailing SYRK update (lower, K=256)")
+        log(f"[bench] rbf mixed: median {medmx:.3f} ms  {paths['rbf_mixed_fp32']['tflops']:.2f} TFLOP/s  "
+            f"eta {eta:.2e}  iters {r['iters']}")
+        assert max(r["iters"]) >= 0 and eta <= TOL, "mixed path did not reach the tolerance"
+        del r
+    del A, B
+    torch.cuda.empty_cache()
+
+    if not args.no_gram:
+        log(f"[bench] gram inputs n={n} nrhs={nrhs}")
+        A, B = make_gram(n, nrhs)
+        torch.cuda.synchronize()
+        r = run_fp64(h, A, B, args.steps, args.warmup)
+        med = median(r["step_ms"])
+        eta = backward_error(h, A, r["X"], B)
+        ferr = factor_error_sampled(A, r["L"])
+        paths["gram_fp64"] = {"tflops": work / (med * 1e-3) / 1e12, "ms": med, "step_ms": r["step_ms"]}
+        acc["gram_fp64"] = {"backward_error": eta, "factor_error_sampled": ferr}
+        log(f"[bench] gram fp64: median {med:.3f} ms  {paths['gram_fp64']['tflops']:.2f} TFLOP/s  eta {eta:.2e}  "
+            f"ferr {ferr:.2e}")
+        del r, A, B
+        torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu:
@@ -183,28 +282,31 @@ def main():
         cpu = cpu_baseline()
         log(f"[bench] cpu: {cpu['value'] * 1e3:.2f} GFLOP/s")
 
+    mixed = "rbf_mixed_fp32" in paths
+    head = paths["rbf_mixed_fp32"] if mixed else paths["rbf_fp64"]
     out = {
         "metric": "potrf+solve TFLOP/s n=16384: FP64 vs FP32-MFMA+refinement @ equal residual",
-        "value": tflops64,
+        "value": head["tflops"],
         "unit": "TFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": med,
-        "wall_ms_per_step": r64["wall_s"] * 1e3 / args.steps,
+        "ms_per_step": head["ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic: A = G G^T/n + 0.1 I, G N(0,1) (seeded torch generator); B U[-1,1]",
-        "config": {"workload": f"dense SPD potrf + potrs, n={n}, nrhs={nrhs}, FP64", "n": n, "nrhs": nrhs,
-                   "path": "fp64"},
-        "accuracy": {"fp64": {"backward_error": eta64, "factor_error_sampled": ferr64}},
-        "paths": {"fp64": {"tflops": tflops64, "ms": med}},
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12 if achieved else None, "peak": PEAK_FP64 / 1e12,
-                     "unit": "TFLOP/s", "frac": (achieved / PEAK_FP64) if achieved else None, "traffic": None,
-                     "kernel": "gemm_kernel (trailing SYRK update, lower trapezoid, K=256)",
-                     "launches": tr["launches"] if tr else None},
+        "dtype": "f32+f64" if mixed else "f64",
+        "data": "synthetic: RBF kernel matrix of 8-d points U[0,1] (lengthscale 0.3, +1e-2 I) and Gram "
+                "G G^T/n + 0.1 I; B U[-1,1]",
+        "config": {"workload": (f"SPD solve (potrf + potrs), RBF matrix n={n}, {nrhs} RHS, "
+                                + ("FP32-MFMA factor + FP64 refinement to backward error <= 1e-14" if mixed
+                                   else "FP64")),
+                   "n": n, "nrhs": nrhs, "tol": TOL},
+        "speedup_mixed_vs_fp64": (paths["rbf_fp64"]["ms"] / paths["rbf_mixed_fp32"]["ms"]) if mixed else None,
+        "paths": paths,
+        "accuracy": acc,
+        "roofline": roof32 if mixed else roof64,
+        "roofline_fp64": roof64,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -148,12 +148,14 @@ rmh_status rmh_set_potrf_blocking(rmh_handle h, size_t nb, size_t nb_inner);
 rmh_status rmh_set_potrf_lookahead(rmh_handle h, int enable);
 
 /* ---------------------------------------------------------- profiling -- */
-/* Kernel classes: bit 0 "trailing_update", bit 1 "panel", bit 2 "diag",
- * bit 3 "inner_update".  While enabled, the handle records HIP events around
- * every launch of a class in class_mask on the stream it is launched on and
- * accumulates the algorithmic flops of each launch (class_mask 0 disables,
- * -1 enables all).  rmh_profile_read waits for the recorded events and
- * returns totals since the last reset. */
+/* Kernel classes of rmh_?potrf: bit 0 "trailing_update" (trailing updates on
+ * 128 x 128 tiles), bit 1 "panel" (the fused panel kernel), bit 2
+ * "trailing_update_small" (trailing updates on 64 x 64 tiles).  While
+ * enabled, the handle records HIP events around every launch of a class in
+ * class_mask on the stream it is launched on and accumulates the algorithmic
+ * flops of each launch (class_mask 0 disables, -1 enables all).
+ * rmh_profile_read waits for the recorded events and returns totals since
+ * the last reset. */
 rmh_status rmh_profile_enable(rmh_handle h, int class_mask);
 rmh_status rmh_profile_reset(rmh_handle h);
 rmh_status rmh_profile_read(rmh_handle h, const char* kernel_class, double* total_ms, double* total_flops,

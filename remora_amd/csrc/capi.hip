@@ -240,46 +240,29 @@ rmh_status trsm_impl(Context* h, bool left, bool upper, bool trans, bool unit, i
 template <typename T>
 rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int64_t* d_info, hipStream_t s) {
     const int64_t nb = (int64_t)h->nb, nbi = (int64_t)h->nbi;
-    RMH_TRY(h->reserve(256));
-    // scratch words: [0] info when the caller passes none, [1] panel reader counter
+    // workspace: [0, 256) info when the caller passes none, then the side
+    // buffer of the diagonal-block factors (n x nbi, ld n)
+    RMH_TRY(h->reserve(256 + (size_t)std::max<int64_t>(n, 1) * nbi * sizeof(T)));
     int64_t* scratch = static_cast<int64_t*>(h->ws);
     int64_t* info = d_info != nullptr ? d_info : scratch;
-    unsigned* readers = reinterpret_cast<unsigned*>(scratch + 1);
+    T* dfac = reinterpret_cast<T*>(static_cast<char*>(h->ws) + 256);
     RMH_TRY(launch_fill_i64(info, 1, 0, s));
-    RMH_TRY(launch_fill_i64(scratch + 1, 1, 0, s));
     if (n == 0) return RMH_STATUS_SUCCESS;
     const int64_t rs = upper ? lda : 1, cs = upper ? 1 : lda;
     auto V = [&](int64_t i, int64_t j) { return A + i * rs + j * cs; };
 
-    // Factor the panel V[k0:, k0:k0+kw] left-looking: before the fused
-    // panel kernel factors block column [j0, j0+jb), one GEMM applies the
-    // updates of the panel's earlier columns [k0, j0) to it, so each block
-    // column is read and written once.
+    // Factor the panel V[k0:, k0:k0+kw] left-looking, one fused kernel per
+    // block column [j0, j0+jb): it applies the updates of the panel's earlier
+    // columns [k0, j0), factors the diagonal block and solves the rows below.
     auto factor_panel = [&](int64_t k0, int64_t kw, hipStream_t st) -> rmh_status {
         for (int64_t j0 = k0; j0 < k0 + kw; j0 += nbi) {
             const int64_t jb = std::min(nbi, k0 + kw - j0);
-            const int64_t rows = n - j0;
+            const int64_t rows = n - j0, K = j0 - k0;
+            const double flops = 2.0 * K * ((double)jb * (jb + 1) / 2.0 + (double)(rows - jb) * jb) +
+                                 (double)jb * jb * jb / 3.0 + (double)(rows - jb) * jb * jb;
             int slot;
-            if (j0 > k0) {
-                const int64_t K = j0 - k0;
-                GemmDesc<T> g;
-                g.alpha = T(-1); g.beta = T(1); g.K = K;
-                if (!upper) {
-                    g.M = rows; g.N = jb; g.part = Part::Lower;
-                    g.A = A + j0 + k0 * lda; g.lda = lda; g.transA = false;
-                    g.B = g.A; g.ldb = lda; g.transB = true;
-                } else {
-                    g.M = jb; g.N = rows; g.part = Part::Upper;
-                    g.A = A + k0 + j0 * lda; g.lda = lda; g.transA = true;
-                    g.B = g.A; g.ldb = lda; g.transB = false;
-                }
-                g.C = A + j0 + j0 * lda; g.ldc = lda;
-                RMH_TRY(h->prof_begin(st, PROF_INNER, 2.0 * K * ((double)jb * (jb + 1) / 2.0 + (double)(rows - jb) * jb), &slot));
-                RMH_TRY(launch_gemm<T>(g, Tile::T64x64, st));
-                RMH_TRY(h->prof_end(st, slot));
-            }
-            RMH_TRY(h->prof_begin(st, PROF_PANEL, (double)jb * jb * jb / 3.0 + (double)(rows - jb) * jb * jb, &slot));
-            RMH_TRY(launch_potrf_panel<T>(V(j0, j0), rs, cs, rows, (int)jb, info, j0, readers, st));
+            RMH_TRY(h->prof_begin(st, PROF_PANEL, flops, &slot));
+            RMH_TRY(launch_potrf_panel<T>(V(j0, k0), rs, cs, rows, K, (int)jb, info, j0, dfac + j0, n, st));
             RMH_TRY(h->prof_end(st, slot));
         }
         return RMH_STATUS_SUCCESS;
@@ -301,9 +284,10 @@ rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
         }
         g.C = A + c0 + c0 * lda; g.ldc = lda;
         const double flops = 2.0 * kw * ((double)cols * (cols + 1) / 2.0 + (double)(rows - cols) * cols);
+        const Tile tile = pick_tile(g.M, g.N, g.part);
         int slot;
-        RMH_TRY(h->prof_begin(st, PROF_TRAILING, flops, &slot));
-        RMH_TRY(launch_gemm<T>(g, pick_tile(g.M, g.N, g.part), st));
+        RMH_TRY(h->prof_begin(st, tile == Tile::T128x128 ? PROF_TRAILING : PROF_TRAILING_SMALL, flops, &slot));
+        RMH_TRY(launch_gemm<T>(g, tile, st));
         return h->prof_end(st, slot);
     };
 
@@ -334,7 +318,7 @@ rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
             RMH_TRY(factor_panel(k1, kw1, s));
         }
     }
-    return RMH_STATUS_SUCCESS;
+    return launch_scatter_diag<T>(A, rs, cs, n, (int)nbi, dfac, n, info, s);
 }
 
 template <typename T>

@@ -8,8 +8,8 @@
 
 namespace rmh {
 
-enum ProfClass { PROF_TRAILING = 0, PROF_PANEL = 1, PROF_DIAG = 2, PROF_INNER = 3, PROF_COUNT = 4 };
-static const char* const kProfNames[PROF_COUNT] = {"trailing_update", "panel", "diag", "inner_update"};
+enum ProfClass { PROF_TRAILING = 0, PROF_PANEL = 1, PROF_TRAILING_SMALL = 2, PROF_COUNT = 3 };
+static const char* const kProfNames[PROF_COUNT] = {"trailing_update", "panel", "trailing_update_small"};
 
 struct ProfRec {
     hipEvent_t a = nullptr, b = nullptr;

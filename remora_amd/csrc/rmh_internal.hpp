@@ -80,14 +80,24 @@ rmh_status launch_symm(const GemmDesc<T>& d, bool upper, hipStream_t stream);
 Tile pick_tile(int64_t M, int64_t N, Part part);
 
 // ------------------------------------------------- small LAPACK kernels --
-// Factor the kb x kb (kb <= 64) diagonal block of a Cholesky panel and
-// solve the rows below it: V(i,c) = A[i*rs + c*cs], rows [0, rows), columns
-// [0, kb), V(0,0) on the diagonal.  On a non-positive pivot j:
-// *info = info_base + j + 1.  Computes in FP64 whatever T is.  *readers is
-// a zero counter in device memory; the kernel leaves it zero again.
+// One block column of a left-looking Cholesky panel step (kb <= 64):
+// V(i,c) = P[i*rs + c*cs], rows [0, rows) from the diagonal down; columns
+// [0, K) are the panel's factored columns, [K, K+kb) the block column.
+// Applies the earlier columns' updates, factors the diagonal block into
+// dfac (element (i, c) at dfac[i + c*ldf]; the block itself keeps its
+// values) and solves the rows below in place.  On a non-positive pivot j:
+// *info = info_base + j + 1 (the first failure is kept).  Computes in FP64
+// whatever T is.
 template <typename T>
-rmh_status launch_potrf_panel(T* A, int64_t rs, int64_t cs, int64_t rows, int kb, int64_t* info, int64_t info_base,
-                              unsigned* readers, hipStream_t stream);
+rmh_status launch_potrf_panel(T* P, int64_t rs, int64_t cs, int64_t rows, int64_t K, int kb, int64_t* info,
+                              int64_t info_base, T* dfac, int64_t ldf, hipStream_t stream);
+
+// Copy the diagonal-block factors written by launch_potrf_panel (blocks of
+// nbi rows; block j0's element (i, c) at dfac[j0 + i + c*ldf]) into V of
+// the n x n matrix; blocks at or after a failed pivot (*info != 0) are left.
+template <typename T>
+rmh_status launch_scatter_diag(T* A, int64_t rs, int64_t cs, int64_t n, int nbi, const T* dfac, int64_t ldf,
+                               const int64_t* info, hipStream_t stream);
 
 // Invert every nb x nb diagonal block of the triangular matrix op(A)
 // (n x n).  eff_upper says whether op(A) is upper triangular.  Block b is

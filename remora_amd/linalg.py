@@ -137,7 +137,7 @@ def set_potrf_lookahead(h, enable=True):
 class Profile:
     """Launch profiler of a handle (HIP events around each launch class)."""
 
-    CLASSES = ("trailing_update", "panel", "diag", "inner_update")
+    CLASSES = ("trailing_update", "panel", "trailing_update_small")
 
     def __init__(self, h):
         self.h = h
