@@ -26,6 +26,7 @@
 // The arithmetic is FP64 whatever the storage type (the mixed-precision
 // factorisation keeps its diagonal blocks in FP64).
 #include "rmh_internal.hpp"
+#include "wave_chol.hpp"
 
 namespace rmh {
 namespace {
@@ -38,14 +39,6 @@ constexpr int kKc = 32;  // K chunk staged in LDS
 constexpr int stride16(int w) { return ((w + 15) / 32) * 32 + 16; }
 
 typedef double dx4 __attribute__((ext_vector_type(4)));
-
-// value of v in lane `lane` (wave-uniform lane index)
-__device__ __forceinline__ double lane_bcast(double v, int lane) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)b, lane);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
 
 // d-th block (bi >= bj) of the lower block triangle, row by row
 __device__ __forceinline__ void lower_block(int d, int& bi, int& bj) {
@@ -222,38 +215,9 @@ __global__ __launch_bounds__(kThreads) void potrf_panel_kernel(T* P, int64_t rs,
     }
     __syncthreads();
 
-    // 2. factor the diagonal block in wavefront 0, lane i owning row i.  The
-    // block is padded to NB x NB with the identity (the factor of diag(A, I)
-    // is diag(L, I)), so every step runs unconditionally; after a
-    // non-positive pivot the remaining steps compute garbage that is never
-    // stored.  Lanes keep garbage in their unused upper part.
+    // 2. factor the (identity-padded) diagonal block in wavefront 0
     if (tid < NB) {
-        int bad = -1;  // wave-uniform
-        double r[NB], piv = 1.0;
-#pragma unroll
-        for (int c = 0; c < NB; ++c) r[c] = L[tid * LD + c];
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
-            const double d = lane_bcast(r[j], j);
-            bad = (bad < 0 && !(d > 0.0)) ? j : bad;  // !(d > 0) also catches NaN
-            // 1/sqrt(d) by the hardware estimate and two Newton steps,
-            // y <- y (3 - d y^2) / 2, then sqrt(d) = d y with one correction
-            // s <- s + (d - s^2) y / 2
-            double y = __builtin_amdgcn_rsq(d);
-            const double hd = 0.5 * d;
-            y = fma(y, fma(-hd * y, y, 0.5), y);
-            y = fma(y, fma(-hd * y, y, 0.5), y);
-            double sj = d * y;
-            sj = fma(fma(-sj, sj, d), 0.5 * y, sj);
-            const double inv = y;
-            r[j] = (tid == j) ? sj : r[j] * inv;
-            piv = (tid == j) ? sj : piv;
-#pragma unroll
-            for (int c = j + 1; c < NB; ++c) r[c] = fma(-r[j], lane_bcast(r[j], c), r[c]);
-        }
-#pragma unroll
-        for (int c = 0; c < NB; ++c) L[tid * LD + c] = r[c];
-        rdiag[tid] = 1.0 / piv;
+        const int bad = wave_potrf<NB, LD>(L, rdiag, tid);
         if (tid == 0) failed = bad;
     }
     __syncthreads();

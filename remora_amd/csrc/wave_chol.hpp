@@ -1,0 +1,56 @@
+// Wavefront-level building blocks of the Cholesky kernels: a lane broadcast
+// and the factorisation of a small diagonal block held in LDS by one
+// wavefront.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace rmh {
+
+// value of v in lane `lane` (wave-uniform lane index)
+__device__ __forceinline__ double lane_bcast(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Right-looking Cholesky of the NB x NB block L (row-major in LDS, ld LD),
+// run by the lanes 0..NB-1 of one wavefront (the caller guards lane < NB):
+// lane i keeps row i in registers and pivot-column entries move lane to lane
+// by readlane.  The block must be padded beyond its true order with the
+// identity (the factor of diag(A, I) is diag(L, I)), so all NB steps run
+// unconditionally.  On return L holds the factor in its lower triangle
+// (garbage above), rdiag[i] = 1 / L_ii, and the result is the first
+// non-positive (or NaN) pivot, -1 if none; after such a pivot the remaining
+// steps compute garbage.  Pivots use the hardware 1/sqrt estimate with two
+// Newton steps, y <- y (3 - d y^2) / 2, and sqrt(d) = d y corrected once by
+// s <- s + (d - s^2) y / 2.
+template <int NB, int LD>
+__device__ __forceinline__ int wave_potrf(double* L, double* rdiag, int lane) {
+    int bad = -1;  // wave-uniform
+    double r[NB], piv = 1.0;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) r[c] = L[lane * LD + c];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const double d = lane_bcast(r[j], j);
+        bad = (bad < 0 && !(d > 0.0)) ? j : bad;
+        double y = __builtin_amdgcn_rsq(d);
+        const double hd = 0.5 * d;
+        y = fma(y, fma(-hd * y, y, 0.5), y);
+        y = fma(y, fma(-hd * y, y, 0.5), y);
+        double sj = d * y;
+        sj = fma(fma(-sj, sj, d), 0.5 * y, sj);
+        r[j] = (lane == j) ? sj : r[j] * y;
+        piv = (lane == j) ? sj : piv;
+#pragma unroll
+        for (int c = j + 1; c < NB; ++c) r[c] = fma(-r[j], lane_bcast(r[j], c), r[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < NB; ++c) L[lane * LD + c] = r[c];
+    rdiag[lane] = 1.0 / piv;
+    return bad;
+}
+
+}  // namespace rmh
