@@ -137,6 +137,63 @@ rmh_status rmh_dsposv(rmh_handle h, int upper, rmh_factor_precision prec, size_t
                       size_t lda, const double* B, size_t ldb, double* X, size_t ldx, double tol, int max_iter,
                       int* iters, double* berr, rmh_stream_t stream);
 
+/* ----------------------------------------------------------- batched -- */
+/* Launch configuration of the batched solver: nb is the block column of the
+ * per-matrix factorisation (16 or 32), wg_per_cu the workgroups per compute
+ * unit of the persistent grid (0: as many as fit), lpt_order whether the
+ * autotuner found a largest-first order (rmh_batched_lpt_order) faster. */
+typedef struct rmh_batch_config {
+    int nb;
+    int wg_per_cu;
+    int lpt_order;
+} rmh_batch_config;
+
+/* Factor and solve `batch` independent SPD systems A_i X_i = B_i in one
+ * persistent launch: A_i is n_i x n_i (0 <= n_i <= max_n <= 512, column-major,
+ * ld lda_i, lower triangle read and overwritten by L_i), B_i is n_i x nrhs
+ * (nrhs <= 16, ld ldb_i, overwritten by X_i).  Every array argument is in
+ * device memory -- d_n, d_lda, d_ldb, d_info (int64 per system), d_A and d_B
+ * (device pointers per system), d_order (optional processing order, a
+ * permutation of 0..batch-1, NULL for 0..batch-1) -- so the launch can be
+ * captured in a graph and replayed with new data.  d_info[i] receives 0, k+1
+ * for a non-positive pivot k (X_i is then not computed), or -1 for n_i out of
+ * range.  cfg NULL uses the handle's tuned configuration.  Asynchronous;
+ * allocates nothing after the handle's first batched call.  No counterpart
+ * in the reference (a new form beside the FP64 drop-in). */
+rmh_status rmh_dposv_batched(rmh_handle h, size_t batch, int max_n, const int64_t* d_n, double* const* d_A,
+                             const int64_t* d_lda, size_t nrhs, double* const* d_B, const int64_t* d_ldb,
+                             int64_t* d_info, const int64_t* d_order, const rmh_batch_config* cfg,
+                             rmh_stream_t stream);
+rmh_status rmh_sposv_batched(rmh_handle h, size_t batch, int max_n, const int64_t* d_n, float* const* d_A,
+                             const int64_t* d_lda, size_t nrhs, float* const* d_B, const int64_t* d_ldb,
+                             int64_t* d_info, const int64_t* d_order, const rmh_batch_config* cfg,
+                             rmh_stream_t stream);
+/* Factor only (rmh_?potrf for every system), same arguments. */
+rmh_status rmh_dpotrf_batched(rmh_handle h, size_t batch, int max_n, const int64_t* d_n, double* const* d_A,
+                              const int64_t* d_lda, int64_t* d_info, const int64_t* d_order,
+                              const rmh_batch_config* cfg, rmh_stream_t stream);
+rmh_status rmh_spotrf_batched(rmh_handle h, size_t batch, int max_n, const int64_t* d_n, float* const* d_A,
+                              const int64_t* d_lda, int64_t* d_info, const int64_t* d_order,
+                              const rmh_batch_config* cfg, rmh_stream_t stream);
+/* Solve only, with factors from rmh_?potrf_batched (rmh_?potrs for every system). */
+rmh_status rmh_dpotrs_batched(rmh_handle h, size_t batch, int max_n, const int64_t* d_n, const double* const* d_A,
+                              const int64_t* d_lda, size_t nrhs, double* const* d_B, const int64_t* d_ldb,
+                              const int64_t* d_order, const rmh_batch_config* cfg, rmh_stream_t stream);
+rmh_status rmh_spotrs_batched(rmh_handle h, size_t batch, int max_n, const int64_t* d_n, const float* const* d_A,
+                              const int64_t* d_lda, size_t nrhs, float* const* d_B, const int64_t* d_ldb,
+                              const int64_t* d_order, const rmh_batch_config* cfg, rmh_stream_t stream);
+/* Largest-first processing order of host sizes h_n into host h_order
+ * (stable for equal sizes). */
+rmh_status rmh_batched_lpt_order(size_t batch, const int64_t* h_n, int64_t* h_order);
+/* Autotuner: times every candidate configuration (nb in {16, 32}, each
+ * power-of-two wg_per_cu up to occupancy, with and without largest-first
+ * order) of rmh_?posv_batched on synthetic SPD systems of the host sizes
+ * h_n with nrhs right-hand sides, stores the fastest in the handle (used
+ * when cfg is NULL) and returns it with its time.  fp64 selects the
+ * precision.  Synchronous; allocates and frees its own test data. */
+rmh_status rmh_batched_autotune(rmh_handle h, int fp64, size_t batch, const int64_t* h_n, size_t nrhs,
+                                rmh_batch_config* best, double* best_ms);
+
 /* ------------------------------------------------------------ tuning -- */
 /* Block sizes of rmh_?potrf: nb is the width of a panel (the K of the
  * trailing update), nb_inner the diagonal block inside a panel (<= 64).

@@ -26,6 +26,11 @@ FACTOR_FP32 = 0
 FACTOR_BF16X3 = 1
 
 
+class BatchConfig(C.Structure):
+    """rmh_batch_config."""
+    _fields_ = [("nb", C.c_int), ("wg_per_cu", C.c_int), ("lpt_order", C.c_int)]
+
+
 class RemoraError(RuntimeError):
     """A non-success rmh_status, with rmh_last_error()'s detail."""
 
@@ -60,6 +65,14 @@ SIGNATURES = {
     "rmh_dpotrs": (ST, [P, I, S, S, P, S, P, S, P]),
     "rmh_spotrs": (ST, [P, I, S, S, P, S, P, S, P]),
     "rmh_dsposv": (ST, [P, I, I, S, S, P, S, P, S, P, S, D, I, C.POINTER(I), C.POINTER(D), P]),
+    "rmh_dposv_batched": (ST, [P, S, I, P, P, P, S, P, P, P, P, P, P]),
+    "rmh_sposv_batched": (ST, [P, S, I, P, P, P, S, P, P, P, P, P, P]),
+    "rmh_dpotrf_batched": (ST, [P, S, I, P, P, P, P, P, P, P]),
+    "rmh_spotrf_batched": (ST, [P, S, I, P, P, P, P, P, P, P]),
+    "rmh_dpotrs_batched": (ST, [P, S, I, P, P, P, S, P, P, P, P, P]),
+    "rmh_spotrs_batched": (ST, [P, S, I, P, P, P, S, P, P, P, P, P]),
+    "rmh_batched_lpt_order": (ST, [S, P, P]),
+    "rmh_batched_autotune": (ST, [P, I, S, P, S, P, C.POINTER(D)]),
     "rmh_set_potrf_blocking": (ST, [P, S, S]),
     "rmh_set_potrf_lookahead": (ST, [P, I]),
     "rmh_profile_enable": (ST, [P, I]),

@@ -1,0 +1,468 @@
+// Batched Cholesky factorisation and solve of many independent SPD systems
+// of uneven order (n_i <= 512) in one persistent launch.
+//
+// No counterpart in the reference (a new form beside the FP64 drop-in); per
+// matrix it runs the same blocked algorithm as rmh_?potrf / rmh_?potrs.
+//
+// Launch: a persistent grid (CUs x workgroups per CU).  Each workgroup takes
+// the next work item from a device counter (in the order a caller-supplied
+// permutation gives, e.g. largest first), factors that matrix and solves its
+// right-hand sides, and repeats.  The last workgroup to leave resets the
+// counters, so consecutive launches (also replays of a captured graph) need
+// no memset.
+//
+// Per matrix (one workgroup, 256 threads), for each NB-wide block column k:
+//   1. the diagonal block is staged in LDS (padded with the identity) and
+//      factored by wavefront 0 (wave_potrf);
+//   2. the rows below are solved against it, one row per thread, and kept in
+//      LDS (P) as the operand of
+//   3. the trailing update A22 -= P P^T on the lower triangle, 16 x 16 MFMA
+//      tiles spread over the four waves, each wave prefetching its next tile.
+// The solve runs block by block: forward with L (diagonal block solved per
+// right-hand side, rows below updated one row per thread), then backward
+// with L^T (rows above updated one row per thread).  The factorisation and
+// the small solves are FP64; the trailing update is on MFMA in T.
+#include <algorithm>
+#include <vector>
+
+#include "rmh_context.hpp"
+#include "wave_chol.hpp"
+
+namespace rmh {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxN = 512;    // largest order of a batched matrix
+constexpr int kMaxRhs = 16;   // largest number of right-hand sides
+
+template <typename T>
+struct BArgs {
+    int64_t batch;
+    const int64_t* n;
+    T* const* A;
+    const int64_t* lda;
+    int nrhs;
+    T* const* B;
+    const int64_t* ldb;
+    int64_t* info;
+    const int64_t* order;
+    unsigned* counters;  // [0] next work item, [1] workgroups done
+    int factor;          // 1: factor A in place; 0: A already holds the factor
+};
+
+template <typename T>
+struct MfmaT;
+template <>
+struct MfmaT<double> {
+    typedef double acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // lane l, item r holds element (row, col) of the 16 x 16 result
+    static __device__ __forceinline__ int row(int l, int r) { return 4 * r + (l >> 4); }
+};
+template <>
+struct MfmaT<float> {
+    typedef float acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
+};
+
+// (ti, tj), ti >= tj, of lower-triangle tile t in row order
+__device__ __forceinline__ void tri_tile(int t, int& ti, int& tj) {
+    int i = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+    while (i * (i + 1) / 2 > t) --i;
+    while ((i + 1) * (i + 2) / 2 <= t) ++i;
+    ti = i;
+    tj = t - i * (i + 1) / 2;
+}
+
+template <typename T, int NB>
+struct Smem {
+    static constexpr int LD = NB + 1;
+    double L[NB * LD];        // diagonal block / its factor
+    double rdiag[NB];
+    T P[kMaxN * LD];          // solved rows below the diagonal block
+    double X[NB * (kMaxRhs + 1)];  // solution block of the solve
+    int bad;
+    int64_t item;
+};
+
+// Factor the n x n matrix A (column-major, ld) in place.  Returns 0 or the
+// LAPACK info (k+1 of the first failing pivot).  Uniform across the block.
+template <typename T, int NB>
+__device__ int64_t factor_one(T* A, int n, int64_t lda, Smem<T, NB>& s, const int tid) {
+    typedef MfmaT<T> M;
+    typedef typename M::acc_t acc_t;
+    constexpr int LD = Smem<T, NB>::LD;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int li = lane & 15, lk = lane >> 4;
+    for (int k = 0; k < n; k += NB) {
+        const int kb = min(NB, n - k), m = n - k - kb;
+        for (int e = tid; e < NB * NB; e += kThreads) {
+            const int i = e % NB, c = e / NB;
+            s.L[i * LD + c] = (i < kb && c <= i) ? (double)A[(k + i) + (int64_t)(k + c) * lda] : (i == c ? 1.0 : 0.0);
+        }
+        __syncthreads();
+        if (tid < NB) {
+            int ln = tid;  // opaque per block column: keeps the lane masks out of the k loop
+            asm volatile("" : "+v"(ln));
+            const int bad = wave_potrf<NB, LD>(s.L, s.rdiag, ln);
+            if (tid == 0) s.bad = bad;
+        }
+        __syncthreads();
+        if (s.bad >= 0) return (int64_t)k + s.bad + 1;
+        for (int e = tid; e < kb * kb; e += kThreads) {
+            const int i = e % kb, c = e / kb;
+            if (c <= i) A[(k + i) + (int64_t)(k + c) * lda] = (T)s.L[i * LD + c];
+        }
+        // rows below: x L^T = a, one row per thread; padded columns come out 0
+        for (int r = tid; r < m; r += kThreads) {
+            T* row = A + (k + kb + r) + (int64_t)k * lda;
+            double x[NB];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) x[c] = (c < kb) ? (double)row[(int64_t)c * lda] : 0.0;
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                double sum[4] = {x[c], 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int t = 0; t < c; ++t) sum[t & 3] = fma(-x[t], s.L[c * LD + t], sum[t & 3]);
+                x[c] = ((sum[0] + sum[1]) + (sum[2] + sum[3])) * s.rdiag[c];
+                __builtin_amdgcn_sched_barrier(0);  // bounds the LDS loads in flight (registers)
+            }
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                if (c < kb) row[(int64_t)c * lda] = (T)x[c];
+                s.P[r * LD + c] = (T)x[c];
+            }
+        }
+        __syncthreads();
+        // trailing update of the lower triangle of A22 (m x m)
+        const int tm = (m + 15) / 16, ntiles = tm * (tm + 1) / 2;
+        T* A22 = A + (k + kb) + (int64_t)(k + kb) * lda;
+        auto load_tile = [&](int t, T (&c)[4], int& ti, int& tj) {
+            tri_tile(t, ti, tj);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 16 * ti + M::row(lane, r), j = 16 * tj + li;
+                c[r] = (i < m && j < m && i >= j) ? A22[i + (int64_t)j * lda] : T(0);
+            }
+        };
+        T cur[4], nxt[4];
+        int ti = 0, tj = 0, nti = 0, ntj = 0;
+        if (wave < ntiles) load_tile(wave, cur, ti, tj);
+        for (int t = wave; t < ntiles; t += 4) {
+            if (t + 4 < ntiles) load_tile(t + 4, nxt, nti, ntj);
+            acc_t acc = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+            for (int q = 0; q < NB / 4; ++q) {
+                const T a = s.P[(16 * ti + li) * LD + 4 * q + lk];
+                const T b = s.P[(16 * tj + li) * LD + 4 * q + lk];
+                acc = M::op(a, b, acc);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 16 * ti + M::row(lane, r), j = 16 * tj + li;
+                if (i < m && j < m && i >= j) A22[i + (int64_t)j * lda] = cur[r] - acc[r];
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) cur[r] = nxt[r];
+            ti = nti;
+            tj = ntj;
+        }
+        __syncthreads();
+    }
+    return 0;
+}
+
+// Solve L L^T X = B in place (B column-major n x nrhs, ld ldb).
+template <typename T, int NB>
+__device__ void solve_one(const T* A, int n, int64_t lda, T* B, int nrhs, int64_t ldb, Smem<T, NB>& s, const int tid) {
+    constexpr int LD = Smem<T, NB>::LD, XL = kMaxRhs + 1;
+    auto load_diag = [&](int k, int kb) {
+        for (int e = tid; e < NB * NB; e += kThreads) {
+            const int i = e % NB, c = e / NB;
+            s.L[i * LD + c] = (i < kb && c <= i) ? (double)A[(k + i) + (int64_t)(k + c) * lda] : (i == c ? 1.0 : 0.0);
+        }
+        for (int e = tid; e < NB * nrhs; e += kThreads) {
+            const int i = e % NB, j = e / NB;
+            s.X[i * XL + j] = (i < kb) ? (double)B[(k + i) + (int64_t)j * ldb] : 0.0;
+        }
+    };
+    // forward: L y = b
+    for (int k = 0; k < n; k += NB) {
+        const int kb = min(NB, n - k), m = n - k - kb;
+        load_diag(k, kb);
+        __syncthreads();
+        if (tid < nrhs) {  // L_kk y = b, one right-hand side per thread, in LDS
+            for (int c = 0; c < kb; ++c) {
+                double sum = s.X[c * XL + tid];
+                for (int t = 0; t < c; ++t) sum = fma(-s.L[c * LD + t], s.X[t * XL + tid], sum);
+                const double y = sum / s.L[c * LD + c];
+                s.X[c * XL + tid] = y;
+                B[(k + c) + (int64_t)tid * ldb] = (T)y;
+            }
+        }
+        __syncthreads();
+        for (int r = tid; r < m; r += kThreads) {  // b_r -= L[r, blk] y_blk
+            const T* lrow = A + (k + kb + r) + (int64_t)k * lda;
+            double l[NB];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) l[c] = (c < kb) ? (double)lrow[(int64_t)c * lda] : 0.0;
+            for (int j = 0; j < nrhs; ++j) {
+                double acc = 0.0;
+#pragma unroll
+                for (int c = 0; c < NB; ++c) acc = fma(l[c], s.X[c * XL + j], acc);
+                T* b = B + (k + kb + r) + (int64_t)j * ldb;
+                *b = (T)((double)*b - acc);
+            }
+        }
+        __syncthreads();
+    }
+    // backward: L^T x = y, last block first
+    const int nblk = (n + NB - 1) / NB;
+    for (int q = nblk - 1; q >= 0; --q) {
+        const int k = q * NB, kb = min(NB, n - k);
+        load_diag(k, kb);
+        __syncthreads();
+        if (tid < nrhs) {  // L_kk^T x = y, one right-hand side per thread, in LDS
+            for (int c = kb - 1; c >= 0; --c) {
+                double sum = s.X[c * XL + tid];
+                for (int t = c + 1; t < kb; ++t) sum = fma(-s.L[t * LD + c], s.X[t * XL + tid], sum);
+                const double x = sum / s.L[c * LD + c];
+                s.X[c * XL + tid] = x;
+                B[(k + c) + (int64_t)tid * ldb] = (T)x;
+            }
+        }
+        __syncthreads();
+        for (int r = tid; r < k; r += kThreads) {  // y_r -= L[blk, r]^T x_blk
+            const T* lcol = A + k + (int64_t)r * lda;  // L(k + c, r), contiguous in c
+            double l[NB];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) l[c] = (c < kb) ? (double)lcol[c] : 0.0;
+            for (int j = 0; j < nrhs; ++j) {
+                double acc = 0.0;
+#pragma unroll
+                for (int c = 0; c < NB; ++c) acc = fma(l[c], s.X[c * XL + j], acc);
+                T* b = B + r + (int64_t)j * ldb;
+                *b = (T)((double)*b - acc);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int NB>
+__global__ __launch_bounds__(kThreads) void batched_posv_kernel(BArgs<T> a) {
+    __shared__ Smem<T, NB> s;
+    const int tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) s.item = (int64_t)atomicAdd(&a.counters[0], 1u);
+        __syncthreads();
+        const int64_t w = s.item;
+        __syncthreads();
+        if (w >= a.batch) break;
+        const int64_t mat = a.order != nullptr ? a.order[w] : w;
+        const int64_t n64 = a.n[mat];
+        int64_t info = 0;
+        if (n64 < 0 || n64 > kMaxN) {
+            info = -1;
+        } else if (n64 > 0) {
+            const int n = (int)n64;
+            // an opaque copy of the thread index: values derived from it are
+            // recomputed per matrix instead of being hoisted out of the
+            // persistent loop (where they would hold registers throughout)
+            int t = tid;
+            asm volatile("" : "+v"(t));
+            if (a.factor) info = factor_one<T, NB>(a.A[mat], n, a.lda[mat], s, t);
+            if (info == 0 && a.nrhs > 0) solve_one<T, NB>(a.A[mat], n, a.lda[mat], a.B[mat], a.nrhs, a.ldb[mat], s, t);
+        }
+        if (tid == 0 && a.info != nullptr) a.info[mat] = info;
+    }
+    // the last workgroup out resets the counters for the next launch
+    if (tid == 0) {
+        __threadfence();
+        if (atomicAdd(&a.counters[1], 1u) == gridDim.x - 1) {
+            atomicExch(&a.counters[0], 0u);
+            atomicExch(&a.counters[1], 0u);
+        }
+    }
+}
+
+template <typename T, int NB>
+rmh_status launch_batched(const BArgs<T>& a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((batched_posv_kernel<T, NB>), dim3((unsigned)grid), dim3(kThreads), 0, s, a);
+    return check_launch("batched_posv_kernel");
+}
+
+template <typename T>
+int max_wg_per_cu(int nb) {
+    int blocks = 0;
+    hipError_t e = nb == 16 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, batched_posv_kernel<T, 16>, kThreads, 0)
+                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, batched_posv_kernel<T, 32>, kThreads, 0);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return std::max(1, blocks);
+}
+
+// synthetic SPD matrices for tuning: a_ij = u(i, j) in [0, 1) symmetric,
+// a_ii = n + 1 (diagonally dominant)
+template <typename T>
+__global__ void synth_spd_kernel(T* const* A, const int64_t* n, const int64_t* lda, int64_t batch) {
+    for (int64_t mat = blockIdx.y; mat < batch; mat += gridDim.y) {
+    const int64_t nn = n[mat];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nn * nn; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e % nn, j = e / nn;
+        const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+        const uint64_t key = (uint64_t)(mat * 1000003 + lo * 7919 + hi);
+        uint64_t z = key + 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+        A[mat][i + j * lda[mat]] = (T)(i == j ? (double)nn + 1.0 : u);
+    }
+    }
+}
+
+}  // namespace
+
+rmh_status Context::batch_counters(unsigned** c) {
+    if (batch_ctr == nullptr) {
+        RMH_TRY_HIP(hipMalloc(&batch_ctr, 256));
+        RMH_TRY_HIP(hipMemset(batch_ctr, 0, 256));  // once, outside any stream capture
+        RMH_TRY_HIP(hipDeviceSynchronize());
+    }
+    *c = static_cast<unsigned*>(batch_ctr);
+    return RMH_STATUS_SUCCESS;
+}
+
+template <typename T>
+rmh_status batched_posv_impl(Context* h, int64_t batch, int max_n, const int64_t* n, T* const* A, const int64_t* lda,
+                             int64_t nrhs, T* const* B, const int64_t* ldb, int64_t* info, const int64_t* order,
+                             const rmh_batch_config* cfg, bool factor, hipStream_t s) {
+    if (batch == 0) return RMH_STATUS_SUCCESS;
+    if (max_n < 0 || max_n > kMaxN) return fail(RMH_STATUS_INVALID_VALUE, "batched: max_n must be in [0, 512]");
+    if (nrhs < 0 || nrhs > kMaxRhs) return fail(RMH_STATUS_INVALID_VALUE, "batched: nrhs must be in [0, 16]");
+    if (nrhs > 0 && (B == nullptr || ldb == nullptr))
+        return fail(RMH_STATUS_INVALID_VALUE, "batched: right-hand sides missing");
+    rmh_batch_config c = cfg != nullptr ? *cfg : h->batch_cfg[sizeof(T) == 8 ? 1 : 0];
+    if (c.nb != 16 && c.nb != 32) return fail(RMH_STATUS_INVALID_VALUE, "batched: block size must be 16 or 32");
+    const int occ = max_wg_per_cu<T>(c.nb);
+    const int per_cu = c.wg_per_cu > 0 ? std::min(c.wg_per_cu, occ) : occ;
+    int cus = 0;
+    RMH_TRY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+    const int grid = (int)std::min<int64_t>(batch, (int64_t)cus * per_cu);
+    unsigned* ctr = nullptr;
+    RMH_TRY(h->batch_counters(&ctr));
+    BArgs<T> a{batch, n, A, lda, (int)nrhs, B, ldb, info, order, ctr, factor ? 1 : 0};
+    return c.nb == 16 ? launch_batched<T, 16>(a, grid, s) : launch_batched<T, 32>(a, grid, s);
+}
+
+// Time each candidate configuration on synthetic matrices of the caller's
+// sizes; keep the fastest in the handle.
+template <typename T>
+rmh_status batched_autotune_impl(Context* h, int64_t batch, const int64_t* h_n, int64_t nrhs, rmh_batch_config* best,
+                                 double* best_ms) {
+    if (batch <= 0) return fail(RMH_STATUS_INVALID_VALUE, "autotune: empty batch");
+    std::vector<int64_t> off(batch + 1, 0), ld(batch), boff(batch + 1, 0);
+    for (int64_t i = 0; i < batch; ++i) {
+        if (h_n[i] < 0 || h_n[i] > kMaxN) return fail(RMH_STATUS_INVALID_VALUE, "autotune: size out of [0, 512]");
+        ld[i] = std::max<int64_t>(1, h_n[i]);
+        off[i + 1] = off[i] + ld[i] * std::max<int64_t>(1, h_n[i]);
+        boff[i + 1] = boff[i] + ld[i] * std::max<int64_t>(1, nrhs);
+    }
+    // device layout: sizes | lds | order | info | A pointers | B pointers | A0 | A | B
+    const size_t el = sizeof(T);
+    size_t bytes = 0;
+    auto take = [&](size_t b) { size_t o = bytes; bytes += (b + 255) / 256 * 256; return o; };
+    const size_t o_n = take(batch * 8), o_ld = take(batch * 8), o_ord = take(batch * 8), o_info = take(batch * 8),
+                 o_pa = take(batch * 8), o_pb = take(batch * 8), o_a0 = take(off[batch] * el), o_a = take(off[batch] * el),
+                 o_b = take(boff[batch] * el);
+    char* base = nullptr;
+    RMH_TRY_HIP(hipMalloc(&base, bytes));
+    struct Free { char* p; ~Free() { (void)hipFree(p); } } guard{base};
+    std::vector<int64_t> order(batch);
+    for (int64_t i = 0; i < batch; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return h_n[x] > h_n[y]; });
+    std::vector<T*> pa(batch), pa0(batch), pb(batch);
+    for (int64_t i = 0; i < batch; ++i) {
+        pa[i] = reinterpret_cast<T*>(base + o_a) + off[i];
+        pa0[i] = reinterpret_cast<T*>(base + o_a0) + off[i];
+        pb[i] = reinterpret_cast<T*>(base + o_b) + boff[i];
+    }
+    RMH_TRY_HIP(hipMemcpy(base + o_n, h_n, batch * 8, hipMemcpyHostToDevice));
+    RMH_TRY_HIP(hipMemcpy(base + o_ld, ld.data(), batch * 8, hipMemcpyHostToDevice));
+    RMH_TRY_HIP(hipMemcpy(base + o_ord, order.data(), batch * 8, hipMemcpyHostToDevice));
+    RMH_TRY_HIP(hipMemcpy(base + o_pa, pa0.data(), batch * 8, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL((synth_spd_kernel<T>), dim3(64, (unsigned)std::min<int64_t>(batch, 65535)), dim3(256), 0, nullptr,
+                       reinterpret_cast<T* const*>(base + o_pa), reinterpret_cast<const int64_t*>(base + o_n),
+                       reinterpret_cast<const int64_t*>(base + o_ld), batch);
+    RMH_TRY(check_launch("synth_spd_kernel"));
+    RMH_TRY_HIP(hipMemcpy(base + o_pa, pa.data(), batch * 8, hipMemcpyHostToDevice));
+    RMH_TRY_HIP(hipMemcpy(base + o_pb, pb.data(), batch * 8, hipMemcpyHostToDevice));
+    hipStream_t s = nullptr;
+    hipEvent_t e0, e1;
+    RMH_TRY_HIP(hipEventCreate(&e0));
+    RMH_TRY_HIP(hipEventCreate(&e1));
+    double best_t = 1e30;
+    rmh_batch_config best_c{32, 0};
+    for (int nb : {16, 32}) {
+        const int occ = max_wg_per_cu<T>(nb);
+        for (int wpc = 1; wpc <= occ; wpc *= 2) {
+            for (int use_order = 0; use_order < 2; ++use_order) {
+                rmh_batch_config c{nb, wpc};
+                float tot = 0.f;
+                const int reps = 3;
+                for (int r = 0; r < reps + 1; ++r) {
+                    RMH_TRY_HIP(hipMemcpyAsync(base + o_a, base + o_a0, off[batch] * el, hipMemcpyDeviceToDevice, s));
+                    RMH_TRY_HIP(hipMemsetAsync(base + o_b, 0, boff[batch] * el, s));
+                    RMH_TRY_HIP(hipEventRecord(e0, s));
+                    RMH_TRY(batched_posv_impl<T>(h, batch, kMaxN, reinterpret_cast<const int64_t*>(base + o_n),
+                                                 reinterpret_cast<T* const*>(base + o_pa),
+                                                 reinterpret_cast<const int64_t*>(base + o_ld), nrhs,
+                                                 reinterpret_cast<T* const*>(base + o_pb),
+                                                 reinterpret_cast<const int64_t*>(base + o_ld),
+                                                 reinterpret_cast<int64_t*>(base + o_info),
+                                                 use_order ? reinterpret_cast<const int64_t*>(base + o_ord) : nullptr, &c,
+                                                 true, s));
+                    RMH_TRY_HIP(hipEventRecord(e1, s));
+                    RMH_TRY_HIP(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    RMH_TRY_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    if (r > 0) tot += ms;
+                }
+                const double t = tot / reps;
+                if (t < best_t) {
+                    best_t = t;
+                    best_c = c;
+                    best_c.lpt_order = use_order;
+                }
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h->batch_cfg[sizeof(T) == 8 ? 1 : 0] = best_c;
+    if (best) *best = best_c;
+    if (best_ms) *best_ms = best_t;
+    return RMH_STATUS_SUCCESS;
+}
+
+template rmh_status batched_posv_impl<double>(Context*, int64_t, int, const int64_t*, double* const*, const int64_t*,
+                                              int64_t, double* const*, const int64_t*, int64_t*, const int64_t*,
+                                              const rmh_batch_config*, bool, hipStream_t);
+template rmh_status batched_posv_impl<float>(Context*, int64_t, int, const int64_t*, float* const*, const int64_t*,
+                                             int64_t, float* const*, const int64_t*, int64_t*, const int64_t*,
+                                             const rmh_batch_config*, bool, hipStream_t);
+template rmh_status batched_autotune_impl<double>(Context*, int64_t, const int64_t*, int64_t, rmh_batch_config*,
+                                                  double*);
+template rmh_status batched_autotune_impl<float>(Context*, int64_t, const int64_t*, int64_t, rmh_batch_config*,
+                                                 double*);
+
+}  // namespace rmh

@@ -124,6 +124,7 @@ Context::~Context() {
     if (ws != nullptr) (void)hipFree(ws);
     for (int i = 0; i < WS_COUNT; ++i)
         if (aux[i] != nullptr) (void)hipFree(aux[i]);
+    if (batch_ctr != nullptr) (void)hipFree(batch_ctr);
 }
 
 // ------------------------------------------------------- TRSM driver --
@@ -591,6 +592,72 @@ rmh_status rmh_dsposv(rmh_handle h, int upper, rmh_factor_precision prec, size_t
     RMH_REQUIRE(A != nullptr && B != nullptr && X != nullptr, "dsposv: NULL pointer");
     return dsposv_impl(ctx(h), upper != 0, prec, (int64_t)n, (int64_t)nrhs, A, (int64_t)lda, B, (int64_t)ldb, X,
                        (int64_t)ldx, tol, max_iter, iters, berr, hs(stream));
+}
+
+// ------------------------------------------------------------ batched --
+rmh_status rmh_dposv_batched(rmh_handle h, size_t batch, int max_n, const int64_t* d_n, double* const* d_A,
+                             const int64_t* d_lda, size_t nrhs, double* const* d_B, const int64_t* d_ldb,
+                             int64_t* d_info, const int64_t* d_order, const rmh_batch_config* cfg,
+                             rmh_stream_t stream) {
+    RMH_CHECK_HANDLE(h);
+    RMH_REQUIRE(batch == 0 || (d_n && d_A && d_lda), "posv_batched: NULL pointer");
+    return batched_posv_impl<double>(ctx(h), (int64_t)batch, max_n, d_n, d_A, d_lda, (int64_t)nrhs, d_B, d_ldb,
+                                     d_info, d_order, cfg, true, hs(stream));
+}
+rmh_status rmh_sposv_batched(rmh_handle h, size_t batch, int max_n, const int64_t* d_n, float* const* d_A,
+                             const int64_t* d_lda, size_t nrhs, float* const* d_B, const int64_t* d_ldb,
+                             int64_t* d_info, const int64_t* d_order, const rmh_batch_config* cfg,
+                             rmh_stream_t stream) {
+    RMH_CHECK_HANDLE(h);
+    RMH_REQUIRE(batch == 0 || (d_n && d_A && d_lda), "posv_batched: NULL pointer");
+    return batched_posv_impl<float>(ctx(h), (int64_t)batch, max_n, d_n, d_A, d_lda, (int64_t)nrhs, d_B, d_ldb, d_info,
+                                    d_order, cfg, true, hs(stream));
+}
+rmh_status rmh_dpotrf_batched(rmh_handle h, size_t batch, int max_n, const int64_t* d_n, double* const* d_A,
+                              const int64_t* d_lda, int64_t* d_info, const int64_t* d_order,
+                              const rmh_batch_config* cfg, rmh_stream_t stream) {
+    RMH_CHECK_HANDLE(h);
+    RMH_REQUIRE(batch == 0 || (d_n && d_A && d_lda), "potrf_batched: NULL pointer");
+    return batched_posv_impl<double>(ctx(h), (int64_t)batch, max_n, d_n, d_A, d_lda, 0, nullptr, nullptr, d_info,
+                                     d_order, cfg, true, hs(stream));
+}
+rmh_status rmh_spotrf_batched(rmh_handle h, size_t batch, int max_n, const int64_t* d_n, float* const* d_A,
+                              const int64_t* d_lda, int64_t* d_info, const int64_t* d_order,
+                              const rmh_batch_config* cfg, rmh_stream_t stream) {
+    RMH_CHECK_HANDLE(h);
+    RMH_REQUIRE(batch == 0 || (d_n && d_A && d_lda), "potrf_batched: NULL pointer");
+    return batched_posv_impl<float>(ctx(h), (int64_t)batch, max_n, d_n, d_A, d_lda, 0, nullptr, nullptr, d_info,
+                                    d_order, cfg, true, hs(stream));
+}
+rmh_status rmh_dpotrs_batched(rmh_handle h, size_t batch, int max_n, const int64_t* d_n, const double* const* d_A,
+                              const int64_t* d_lda, size_t nrhs, double* const* d_B, const int64_t* d_ldb,
+                              const int64_t* d_order, const rmh_batch_config* cfg, rmh_stream_t stream) {
+    RMH_CHECK_HANDLE(h);
+    RMH_REQUIRE(batch == 0 || (d_n && d_A && d_lda), "potrs_batched: NULL pointer");
+    // the solve only reads A (factor == false)
+    return batched_posv_impl<double>(ctx(h), (int64_t)batch, max_n, d_n, const_cast<double* const*>(d_A), d_lda,
+                                     (int64_t)nrhs, d_B, d_ldb, nullptr, d_order, cfg, false, hs(stream));
+}
+rmh_status rmh_spotrs_batched(rmh_handle h, size_t batch, int max_n, const int64_t* d_n, const float* const* d_A,
+                              const int64_t* d_lda, size_t nrhs, float* const* d_B, const int64_t* d_ldb,
+                              const int64_t* d_order, const rmh_batch_config* cfg, rmh_stream_t stream) {
+    RMH_CHECK_HANDLE(h);
+    RMH_REQUIRE(batch == 0 || (d_n && d_A && d_lda), "potrs_batched: NULL pointer");
+    return batched_posv_impl<float>(ctx(h), (int64_t)batch, max_n, d_n, const_cast<float* const*>(d_A), d_lda,
+                                    (int64_t)nrhs, d_B, d_ldb, nullptr, d_order, cfg, false, hs(stream));
+}
+rmh_status rmh_batched_lpt_order(size_t batch, const int64_t* h_n, int64_t* h_order) {
+    RMH_REQUIRE(batch == 0 || (h_n && h_order), "batched_lpt_order: NULL pointer");
+    for (size_t i = 0; i < batch; ++i) h_order[i] = (int64_t)i;
+    std::stable_sort(h_order, h_order + batch, [&](int64_t a, int64_t b) { return h_n[a] > h_n[b]; });
+    return RMH_STATUS_SUCCESS;
+}
+rmh_status rmh_batched_autotune(rmh_handle h, int fp64, size_t batch, const int64_t* h_n, size_t nrhs,
+                                rmh_batch_config* best, double* best_ms) {
+    RMH_CHECK_HANDLE(h);
+    RMH_REQUIRE(h_n != nullptr, "batched_autotune: NULL sizes");
+    if (fp64) return batched_autotune_impl<double>(ctx(h), (int64_t)batch, h_n, (int64_t)nrhs, best, best_ms);
+    return batched_autotune_impl<float>(ctx(h), (int64_t)batch, h_n, (int64_t)nrhs, best, best_ms);
 }
 
 rmh_status rmh_set_potrf_lookahead(rmh_handle h, int enable) {

@@ -36,6 +36,10 @@ struct Context {
     hipStream_t side = nullptr;  // look-ahead stream of potrf (default priority)
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     std::vector<ProfRec> recs;
+    // batched solver: tuned launch configuration ([0] FP32, [1] FP64) and the
+    // work counters of its persistent kernel (zeroed once, kept zero by it)
+    rmh_batch_config batch_cfg[2] = {{32, 0, 1}, {16, 0, 1}};
+    void* batch_ctr = nullptr;
 
     rmh_status activate();
     rmh_status reserve(size_t bytes);
@@ -44,6 +48,7 @@ struct Context {
     rmh_status prof_begin(hipStream_t s, int cls, double flops, int* slot);
     rmh_status prof_end(hipStream_t s, int slot);
     void prof_clear();
+    rmh_status batch_counters(unsigned** c);
     ~Context();
 };
 
@@ -63,5 +68,13 @@ rmh_status potrs_impl(Context* h, bool upper, int64_t n, int64_t nrhs, const T* 
 rmh_status dsposv_impl(Context* h, bool upper, rmh_factor_precision prec, int64_t n, int64_t nrhs, const double* A,
                        int64_t lda, const double* B, int64_t ldb, double* X, int64_t ldx, double tol, int max_iter,
                        int* iters, double* berr, hipStream_t s);
+
+template <typename T>
+rmh_status batched_posv_impl(Context* h, int64_t batch, int max_n, const int64_t* n, T* const* A, const int64_t* lda,
+                             int64_t nrhs, T* const* B, const int64_t* ldb, int64_t* info, const int64_t* order,
+                             const rmh_batch_config* cfg, bool factor, hipStream_t s);
+template <typename T>
+rmh_status batched_autotune_impl(Context* h, int64_t batch, const int64_t* h_n, int64_t nrhs, rmh_batch_config* best,
+                                 double* best_ms);
 
 }  // namespace rmh

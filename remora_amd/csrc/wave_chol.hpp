@@ -45,7 +45,13 @@ __device__ __forceinline__ int wave_potrf(double* L, double* rdiag, int lane) {
         r[j] = (lane == j) ? sj : r[j] * y;
         piv = (lane == j) ? sj : piv;
 #pragma unroll
-        for (int c = j + 1; c < NB; ++c) r[c] = fma(-r[j], lane_bcast(r[j], c), r[c]);
+        for (int c = j + 1; c < NB; ++c) {
+            r[c] = fma(-r[j], lane_bcast(r[j], c), r[c]);
+            // keep the scheduler from hoisting many broadcasts at once: they
+            // would pile up in scalar registers and spill
+            if ((c & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
     for (int c = 0; c < NB; ++c) L[lane * LD + c] = r[c];

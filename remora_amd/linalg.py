@@ -126,6 +126,78 @@ def dsposv(h, A, B, X, upper=False, prec="fp32", tol=1e-14, max_iter=30, stream=
     return it.value, be.value
 
 
+class Batch:
+    """Device-side description of a batch of column-major SPD systems.
+
+    ``As`` and ``Bs`` are lists of column-major device tensors (``colmajor``);
+    the pointer, size and leading-dimension arrays are built once on the
+    device, so a launch over them can be captured and replayed.
+    """
+
+    def __init__(self, As, Bs=None, order=None):
+        dev = As[0].device
+        self.As, self.Bs = As, Bs
+        self.batch = len(As)
+        self.dtype = As[0].dtype
+        self.max_n = max(int(a.shape[0]) for a in As)
+        self.n = torch.tensor([a.shape[0] for a in As], dtype=torch.int64, device=dev)
+        self.lda = torch.tensor([max(1, _ld(a)) for a in As], dtype=torch.int64, device=dev)
+        self.pa = torch.tensor([a.data_ptr() for a in As], dtype=torch.int64, device=dev)
+        self.info = torch.zeros(self.batch, dtype=torch.int64, device=dev)
+        self.nrhs = 0
+        if Bs is not None:
+            self.nrhs = int(Bs[0].shape[1])
+            self.ldb = torch.tensor([max(1, _ld(b)) for b in Bs], dtype=torch.int64, device=dev)
+            self.pb = torch.tensor([b.data_ptr() for b in Bs], dtype=torch.int64, device=dev)
+        self.order = None if order is None else torch.as_tensor(order, dtype=torch.int64).to(dev)
+
+    def _cfg(self, cfg):
+        return None if cfg is None else C.byref(cfg)
+
+
+def _bp(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def posv_batched(h, batch: Batch, cfg=None, stream=None):
+    f = _fn("rmh_", batch.dtype, "posv_batched")
+    capi.check(f(h.value, batch.batch, batch.max_n, _bp(batch.n), _bp(batch.pa), _bp(batch.lda), batch.nrhs,
+                 _bp(batch.pb), _bp(batch.ldb), _bp(batch.info), _bp(batch.order), batch._cfg(cfg), _stream(stream)))
+    return batch.info
+
+
+def potrf_batched(h, batch: Batch, cfg=None, stream=None):
+    f = _fn("rmh_", batch.dtype, "potrf_batched")
+    capi.check(f(h.value, batch.batch, batch.max_n, _bp(batch.n), _bp(batch.pa), _bp(batch.lda), _bp(batch.info),
+                 _bp(batch.order), batch._cfg(cfg), _stream(stream)))
+    return batch.info
+
+
+def potrs_batched(h, batch: Batch, cfg=None, stream=None):
+    f = _fn("rmh_", batch.dtype, "potrs_batched")
+    capi.check(f(h.value, batch.batch, batch.max_n, _bp(batch.n), _bp(batch.pa), _bp(batch.lda), batch.nrhs,
+                 _bp(batch.pb), _bp(batch.ldb), _bp(batch.order), batch._cfg(cfg), _stream(stream)))
+
+
+def lpt_order(sizes):
+    import numpy as np
+    n = np.ascontiguousarray(np.asarray(sizes, dtype=np.int64))
+    out = np.empty_like(n)
+    capi.check(capi.load().rmh_batched_lpt_order(len(n), n.ctypes.data, out.ctypes.data))
+    return out
+
+
+def batched_autotune(h, sizes, nrhs, fp64=True):
+    """Run the autotuner on host sizes; returns (BatchConfig, best ms)."""
+    import numpy as np
+    n = np.ascontiguousarray(np.asarray(sizes, dtype=np.int64))
+    cfg = capi.BatchConfig()
+    ms = C.c_double()
+    capi.check(capi.load().rmh_batched_autotune(h.value, int(fp64), len(n), n.ctypes.data, nrhs, C.byref(cfg),
+                                                C.byref(ms)))
+    return cfg, ms.value
+
+
 def set_potrf_blocking(h, nb=0, nb_inner=0):
     capi.check(capi.load().rmh_set_potrf_blocking(h.value, nb, nb_inner))
 
