@@ -164,6 +164,98 @@ def run_mixed(h, A, B, steps, warmup, prec="fp32", profile=True):
             "berr": [b for _, b in hist[warmup:]]}
 
 
+def make_batched(count=4096, lo=16, hi=512, nrhs=8, dtype=None, seed=2):
+    """4096 RBF systems: n_i uniform in [16, 512] (project RNG, seed 2), points
+    n_i x 4 in [0,1] (own stream per system), lengthscale 0.5, + 1e-3 I; B
+    U[-1,1] with 8 columns.  Built on the GPU into two packed buffers."""
+    import torch
+    from remora_amd import synth
+    dtype = dtype or torch.float64
+    sizes = [int(n) for n in synth.batched_sizes(count, lo, hi, seed=seed)]
+    offs, boffs, tot, btot = [], [], 0, 0
+    for n in sizes:
+        offs.append(tot)
+        boffs.append(btot)
+        tot += n * n
+        btot += n * nrhs
+    A0 = torch.empty(tot, dtype=torch.float64, device="cuda")
+    B0 = torch.empty(btot, dtype=torch.float64, device="cuda")
+    for i, n in enumerate(sizes):
+        x = torch.from_numpy(synth.points(n, 4, seed=seed, stream=100 + i)).cuda()
+        sq = (x * x).sum(dim=1)
+        d2 = (sq[:, None] + sq[None, :] - 2.0 * (x @ x.t())).clamp_min_(0.0)
+        a = torch.exp(d2 * (-1.0 / (2.0 * 0.5 * 0.5)))
+        a = 0.5 * (a + a.t())
+        a.diagonal().add_(1e-3)
+        A0[offs[i]:offs[i] + n * n].copy_(a.t().reshape(-1))  # column-major
+        B0[boffs[i]:boffs[i] + n * nrhs].copy_(
+            torch.from_numpy(synth.rhs(n, nrhs, seed=seed + 1, stream=200 + i)).cuda().t().reshape(-1))
+    return sizes, offs, boffs, A0.to(dtype), B0.to(dtype)
+
+
+def run_batched(h, steps, warmup, dtype_name):
+    """Autotune, then time rmh_?posv_batched (factor + 8-RHS solve of all
+    4096 systems in one persistent launch, replayed from a captured graph)."""
+    import torch
+    from remora_amd import linalg as la
+    dtype = torch.float64 if dtype_name == "f64" else torch.float32
+    nrhs = 8
+    sizes, offs, boffs, A0, B0 = make_batched(nrhs=nrhs, dtype=dtype)
+    A, B = A0.clone(), B0.clone()
+    As = [A[o:o + n * n].view(n, n).t() for o, n in zip(offs, sizes)]
+    Bs = [B[o:o + n * nrhs].view(nrhs, n).t() for o, n in zip(boffs, sizes)]
+    cfg, tune_ms = la.batched_autotune(h, sizes, nrhs, fp64=(dtype_name == "f64"))
+    order = la.lpt_order(sizes) if cfg.lpt_order else None
+    batch = la.Batch(As, Bs, order=order)
+    la.posv_batched(h, batch)  # eager once
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        la.posv_batched(h, batch)
+    times = []
+    for i in range(warmup + steps):
+        A.copy_(A0)
+        B.copy_(B0)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        g.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        if i >= warmup:
+            times.append(e0.elapsed_time(e1))
+    info = batch.info.cpu()
+    assert int((info != 0).sum()) == 0, "batched: a system failed to factor"
+    # accuracy on a sample of systems (host FP64 check)
+    import numpy as np
+    from remora_amd import synth
+    worst_eta, worst_ferr = 0.0, 0.0
+    for i in list(range(0, len(sizes), 512)) + [int(np.argmax(sizes))]:
+        n = sizes[i]
+        a = A0[offs[i]:offs[i] + n * n].double().view(n, n).t().cpu().numpy()
+        b = B0[boffs[i]:boffs[i] + n * nrhs].double().view(nrhs, n).t().cpu().numpy()
+        L = np.tril(As[i].double().cpu().numpy())
+        x = Bs[i].double().cpu().numpy()
+        worst_eta = max(worst_eta, synth.backward_error(a, x, b))
+        worst_ferr = max(worst_ferr, float(np.linalg.norm(a - L @ L.T) / np.linalg.norm(a)))
+    ms = median(times)
+    n_arr = np.asarray(sizes, dtype=np.float64)
+    work = float(np.sum(n_arr ** 3 / 3 + 2 * n_arr ** 2 * nrhs))
+    esize = 8 if dtype_name == "f64" else 4
+    nbytes = float(np.sum(2 * n_arr ** 2 + 2 * nrhs * n_arr)) * esize
+    achieved = work / (ms * 1e-3)
+    peak = PEAK[dtype_name]
+    ceiling = min(peak, 8e12 * work / nbytes)
+    return {"tflops": achieved / 1e12, "ms": ms, "step_ms": times, "systems": len(sizes), "nrhs": nrhs,
+            "work_flops": work, "algorithmic_bytes": nbytes,
+            "config": {"nb": cfg.nb, "wg_per_cu": cfg.wg_per_cu, "lpt_order": cfg.lpt_order, "autotune_ms": tune_ms},
+            "accuracy": {"max_backward_error_sampled": worst_eta, "max_factor_error_sampled": worst_ferr},
+            "roofline": {"bound": "hbm" if ceiling < peak else "mfma", "achieved": achieved / 1e12,
+                         "peak": ceiling / 1e12, "unit": "TFLOP/s", "frac": achieved / ceiling,
+                         "intensity_flop_per_byte": work / nbytes}}
+
+
 def median(v):
     s = sorted(v)
     return s[len(s) // 2]
@@ -209,6 +301,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-gram", action="store_true", help="skip the FP64 Gram-matrix leg")
     ap.add_argument("--only-fp64", action="store_true", help="FP64 legs only (no mixed leg)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-systems legs")
     ap.add_argument("--nb", type=int, default=0, help="potrf panel width (0: library default)")
     ap.add_argument("--nbi", type=int, default=0, help="potrf diagonal block (0: library default)")
     ap.add_argument("--no-lookahead", action="store_true", help="single-stream factorisation")
@@ -275,6 +368,16 @@ def main():
             f"ferr {ferr:.2e}")
         del r, A, B
         torch.cuda.empty_cache()
+
+    if not args.no_batched:
+        for dt in ("f64", "f32"):
+            log(f"[bench] batched {dt}: 4096 RBF systems, n in [16, 512], 8 RHS")
+            r = run_batched(h, args.steps, args.warmup, dt)
+            paths[f"batched_{dt}"] = r
+            log(f"[bench] batched {dt}: median {r['ms']:.3f} ms  {r['tflops']:.2f} TFLOP/s  "
+                f"({100 * r['roofline']['frac']:.1f}% of {r['roofline']['peak']:.1f})  cfg {r['config']}  "
+                f"eta {r['accuracy']['max_backward_error_sampled']:.2e}")
+            torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu:
