@@ -256,6 +256,51 @@ def run_batched(h, steps, warmup, dtype_name):
                          "intensity_flop_per_byte": work / nbytes}}
 
 
+def run_adjoint(h, steps, warmup, n=8192):
+    """GP negative log marginal likelihood at n=8192 (RBF of points in
+    [0,1]^8, lengthscale 0.3, + 1e-2 I; y N(0,1)): forward potrf, the factor's
+    gradient Lbar = tril(-w z^T) + diag(1/L_ii) (z = L^-1 y, w = L^-T z), then
+    the timed step: rmh_dpotrf_adjoint.  Work 7n^3/3 (the L^T Lbar product
+    on its triangle, n^3/3, and two n x n triangular solves, n^3 each)."""
+    import torch
+    from remora_amd import linalg as la
+    from remora_amd import synth
+    A, _ = make_rbf(n, 1, d=8, lengthscale=0.3, jitter=1e-2, seed=1)
+    y = torch.from_numpy(synth.normal(11, 0, n)).cuda()
+    L = la.colmajor(A.clone())
+    info = la.potrf(h, L)
+    assert int(info.item()) == 0
+    Lt = torch.tril(L)
+    z = torch.linalg.solve_triangular(Lt, y[:, None], upper=False)
+    w = torch.linalg.solve_triangular(Lt.t(), z, upper=True)
+    Lbar = la.colmajor(torch.tril(-w @ z.t()))
+    Lbar.diagonal().add_(1.0 / Lt.diagonal())
+    Abar = la.empty_colmajor(n, n)
+    times = []
+    for i in range(warmup + steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        la.potrf_adjoint(h, L, Lbar, Abar)
+        e1.record()
+        torch.cuda.synchronize()
+        if i >= warmup:
+            times.append(e0.elapsed_time(e1))
+    # closed form 0.5 (A^-1 - alpha alpha^T) with the library's own solve
+    Ainv = la.colmajor(torch.eye(n, dtype=torch.float64, device="cuda"))
+    la.potrs(h, L, Ainv)
+    alpha = Ainv @ y
+    ref = 0.5 * (Ainv - torch.outer(alpha, alpha))
+    ref = 0.5 * (ref + ref.t())
+    err = float(((Abar - ref).abs().max() / ref.abs().max()).item())
+    ms = median(times)
+    work = 7.0 * n ** 3 / 3.0
+    achieved = work / (ms * 1e-3)
+    return {"tflops": achieved / 1e12, "ms": ms, "step_ms": times, "n": n, "work_flops": work,
+            "accuracy": {"max_rel_err_vs_closed_form": err},
+            "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["f64"] / 1e12, "unit": "TFLOP/s",
+                         "frac": achieved / PEAK["f64"]}}
+
+
 def median(v):
     s = sorted(v)
     return s[len(s) // 2]
@@ -302,6 +347,7 @@ def main():
     ap.add_argument("--no-gram", action="store_true", help="skip the FP64 Gram-matrix leg")
     ap.add_argument("--only-fp64", action="store_true", help="FP64 legs only (no mixed leg)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-systems legs")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the Cholesky-adjoint leg")
     ap.add_argument("--nb", type=int, default=0, help="potrf panel width (0: library default)")
     ap.add_argument("--nbi", type=int, default=0, help="potrf diagonal block (0: library default)")
     ap.add_argument("--no-lookahead", action="store_true", help="single-stream factorisation")
@@ -378,6 +424,14 @@ def main():
                 f"({100 * r['roofline']['frac']:.1f}% of {r['roofline']['peak']:.1f})  cfg {r['config']}  "
                 f"eta {r['accuracy']['max_backward_error_sampled']:.2e}")
             torch.cuda.empty_cache()
+
+    if not args.no_adjoint:
+        log("[bench] adjoint: GP NLML gradient, n=8192")
+        r = run_adjoint(h, args.steps, args.warmup)
+        paths["adjoint_f64"] = r
+        log(f"[bench] adjoint: median {r['ms']:.3f} ms  {r['tflops']:.2f} TFLOP/s  "
+            f"err vs closed form {r['accuracy']['max_rel_err_vs_closed_form']:.2e}")
+        torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu:

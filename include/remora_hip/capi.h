@@ -137,6 +137,21 @@ rmh_status rmh_dsposv(rmh_handle h, int upper, rmh_factor_precision prec, size_t
                       size_t lda, const double* B, size_t ldb, double* X, size_t ldx, double tol, int max_iter,
                       int* iters, double* berr, rmh_stream_t stream);
 
+/* ------------------------------------------------------ differentiation -- */
+/* Reverse-mode derivative of the Cholesky factorisation: given the factor
+ * from rmh_?potrf (L lower, or U = L^T upper; only that triangle read) and
+ * the gradient Lbar of a scalar with respect to that triangle, writes the
+ * symmetric gradient with respect to A to Abar (n x n, both triangles):
+ *     Abar = sym(L^-T Phi(L^T Lbar) L^-1),  Phi = lower triangle, diagonal halved,
+ * sym(G) = (G + G^T)/2.  The inverses of L's diagonal blocks are recomputed
+ * inside the two triangular solves, not stored.  Abar must not alias L or
+ * Lbar.  Asynchronous.  No counterpart in the reference (a new form beside the
+ * FP64 drop-in). */
+rmh_status rmh_dpotrf_adjoint(rmh_handle h, int upper, size_t n, const double* L, size_t ldl, const double* Lbar,
+                              size_t ldlb, double* Abar, size_t ldab, rmh_stream_t stream);
+rmh_status rmh_spotrf_adjoint(rmh_handle h, int upper, size_t n, const float* L, size_t ldl, const float* Lbar,
+                              size_t ldlb, float* Abar, size_t ldab, rmh_stream_t stream);
+
 /* ----------------------------------------------------------- batched -- */
 /* Launch configuration of the batched solver: nb is the block column of the
  * per-matrix factorisation (16 or 32), wg_per_cu the workgroups per compute

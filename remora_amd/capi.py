@@ -65,6 +65,8 @@ SIGNATURES = {
     "rmh_dpotrs": (ST, [P, I, S, S, P, S, P, S, P]),
     "rmh_spotrs": (ST, [P, I, S, S, P, S, P, S, P]),
     "rmh_dsposv": (ST, [P, I, I, S, S, P, S, P, S, P, S, D, I, C.POINTER(I), C.POINTER(D), P]),
+    "rmh_dpotrf_adjoint": (ST, [P, I, S, P, S, P, S, P, S, P]),
+    "rmh_spotrf_adjoint": (ST, [P, I, S, P, S, P, S, P, S, P]),
     "rmh_dposv_batched": (ST, [P, S, I, P, P, P, S, P, P, P, P, P, P]),
     "rmh_sposv_batched": (ST, [P, S, I, P, P, P, S, P, P, P, P, P, P]),
     "rmh_dpotrf_batched": (ST, [P, S, I, P, P, P, P, P, P, P]),

@@ -456,6 +456,17 @@ rmh_status gemv_entry(rmh_handle h, int trans, size_t m, size_t n, T alpha, cons
 }
 
 template <typename T>
+rmh_status adjoint_entry(rmh_handle h, int upper, size_t n, const T* L, size_t ldl, const T* Lbar, size_t ldlb, T* Abar,
+                         size_t ldab, rmh_stream_t stream) {
+    RMH_CHECK_HANDLE(h);
+    RMH_REQUIRE(ldl >= mx1(n) && ldlb >= mx1(n) && ldab >= mx1(n), "potrf_adjoint: leading dimension too small");
+    if (n == 0) return RMH_STATUS_SUCCESS;
+    RMH_REQUIRE(L != nullptr && Lbar != nullptr && Abar != nullptr, "potrf_adjoint: NULL pointer");
+    RMH_REQUIRE(Abar != L && Abar != Lbar, "potrf_adjoint: Abar aliases an input");
+    return potrf_adjoint_impl<T>(ctx(h), upper != 0, (int64_t)n, L, (int64_t)ldl, Lbar, (int64_t)ldlb, Abar,
+                                 (int64_t)ldab, hs(stream));
+}
+template <typename T>
 rmh_status potrf_entry(rmh_handle h, int upper, size_t n, T* A, size_t lda, int64_t* d_info, rmh_stream_t stream) {
     RMH_CHECK_HANDLE(h);
     RMH_REQUIRE(lda >= mx1(n), "potrf: lda too small");
@@ -592,6 +603,16 @@ rmh_status rmh_dsposv(rmh_handle h, int upper, rmh_factor_precision prec, size_t
     RMH_REQUIRE(A != nullptr && B != nullptr && X != nullptr, "dsposv: NULL pointer");
     return dsposv_impl(ctx(h), upper != 0, prec, (int64_t)n, (int64_t)nrhs, A, (int64_t)lda, B, (int64_t)ldb, X,
                        (int64_t)ldx, tol, max_iter, iters, berr, hs(stream));
+}
+
+// ----------------------------------------------------- differentiation --
+rmh_status rmh_dpotrf_adjoint(rmh_handle h, int upper, size_t n, const double* L, size_t ldl, const double* Lbar,
+                              size_t ldlb, double* Abar, size_t ldab, rmh_stream_t stream) {
+    return adjoint_entry<double>(h, upper, n, L, ldl, Lbar, ldlb, Abar, ldab, stream);
+}
+rmh_status rmh_spotrf_adjoint(rmh_handle h, int upper, size_t n, const float* L, size_t ldl, const float* Lbar,
+                              size_t ldlb, float* Abar, size_t ldab, rmh_stream_t stream) {
+    return adjoint_entry<float>(h, upper, n, L, ldl, Lbar, ldlb, Abar, ldab, stream);
 }
 
 // ------------------------------------------------------------ batched --

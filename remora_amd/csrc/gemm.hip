@@ -79,6 +79,9 @@ struct KParams {
     // its partial product to split_ws + y*M*N (ld M); a reduction combines them
     int64_t k_chunk;
     T* split_ws;
+    // triangular operands: a tile whose rows start at m0 only accumulates
+    // k >= m0 (op(A)(i, k) = 0 for k < i)
+    int k_from_row;
 };
 
 constexpr int kThreads = 256;
@@ -162,6 +165,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         lower_trapezoid_tile(pid, p.tiles_m, tn, tm);
     }
     const int64_t m0 = tm * BM, n0 = tn * BN;
+    if (p.k_from_row) {
+        const int64_t kb = min(p.K, m0);  // m0 is a multiple of BK
+        p.A += TA ? kb : kb * p.lda;
+        p.B += TB ? kb * p.ldb : kb;
+        p.K -= kb;
+    }
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -364,6 +373,7 @@ rmh_status launch_cfg(const GemmDesc<T>& d, hipStream_t stream) {
     p.sAy = d.sAy; p.sAz = d.sAz; p.sBy = d.sBy; p.sBz = d.sBz; p.sCy = d.sCy; p.sCz = d.sCz;
     p.k_chunk = d.k_chunk;
     p.split_ws = d.split_ws;
+    p.k_from_row = d.k_from_row ? 1 : 0;
     int64_t ntiles;
     if (PART == 0) {
         ntiles = p.tiles_m * p.tiles_n;
@@ -420,6 +430,7 @@ rmh_status launch_symm_cfg(const GemmDesc<T>& d, hipStream_t stream) {
     p.batch_ny = 1;
     p.sAy = p.sAz = p.sBy = p.sBz = p.sCy = p.sCz = 0;
     p.k_chunk = d.K;
+    p.k_from_row = 0;
     p.split_ws = nullptr;
     const int64_t ntiles = p.tiles_m * p.tiles_n;
     if (ntiles <= 0) return RMH_STATUS_SUCCESS;
@@ -459,6 +470,8 @@ template <typename T>
 rmh_status launch_gemm(const GemmDesc<T>& d_in, Tile tile, hipStream_t stream) {
     GemmDesc<T> d = d_in;
     if (d.M <= 0 || d.N <= 0) return RMH_STATUS_SUCCESS;
+    if (d.k_from_row && (d.splits > 1 || d.batch > 1))
+        return fail(RMH_STATUS_INVALID_VALUE, "gemm: triangular depth skip needs one plain launch");
     if (d.splits > 1) {
         if (d.split_ws == nullptr || d.part != Part::Full || d.batch > 1 || d.K <= 0)
             return fail(RMH_STATUS_INVALID_VALUE, "gemm: split-K needs a workspace and a plain rectangle");

@@ -19,7 +19,7 @@ struct ProfRec {
 
 // Scratch slots of a handle: each grows on demand and is reused by every
 // call that uses it.  Routines that call other routines use different slots.
-enum WsSlot { WS_MAIN = 0, WS_MIXED = 1, WS_FACTOR = 2, WS_BATCH = 3, WS_COUNT = 4 };
+enum WsSlot { WS_MAIN = 0, WS_MIXED = 1, WS_FACTOR = 2, WS_ADJOINT = 3, WS_COUNT = 4 };
 
 struct Context {
     static constexpr size_t kDefaultNb = 256;
@@ -76,5 +76,9 @@ rmh_status batched_posv_impl(Context* h, int64_t batch, int max_n, const int64_t
 template <typename T>
 rmh_status batched_autotune_impl(Context* h, int64_t batch, const int64_t* h_n, int64_t nrhs, rmh_batch_config* best,
                                  double* best_ms);
+
+template <typename T>
+rmh_status potrf_adjoint_impl(Context* h, bool upper, int64_t n, const T* L, int64_t ldl, const T* Lbar, int64_t ldlb,
+                              T* Abar, int64_t ldab, hipStream_t s);
 
 }  // namespace rmh

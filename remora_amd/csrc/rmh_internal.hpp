@@ -64,6 +64,9 @@ struct GemmDesc {
     int splits = 1;
     T* split_ws = nullptr;
     int64_t k_chunk = 0;  // set by launch_gemm
+    // op(A) is upper triangular (op(A)(i, k) = 0 for k < i): each tile skips
+    // the depth below its first row (no split-K, no batch)
+    bool k_from_row = false;
 };
 
 // Launch C = alpha*op(A)*op(B) + beta*C (restricted to `part`).  `tile`

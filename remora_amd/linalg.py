@@ -126,6 +126,17 @@ def dsposv(h, A, B, X, upper=False, prec="fp32", tol=1e-14, max_iter=30, stream=
     return it.value, be.value
 
 
+def potrf_adjoint(h, L, Lbar, Abar=None, upper=False, stream=None):
+    """Abar = sym(L^-T Phi(L^T Lbar) L^-1): gradient w.r.t. A of a scalar
+    whose gradient w.r.t. the Cholesky factor is Lbar."""
+    n = L.shape[0]
+    if Abar is None:
+        Abar = empty_colmajor(n, n, dtype=L.dtype, device=L.device)
+    f = _fn("rmh_", L.dtype, "potrf_adjoint")
+    capi.check(f(h.value, int(upper), n, _p(L), _ld(L), _p(Lbar), _ld(Lbar), _p(Abar), _ld(Abar), _stream(stream)))
+    return Abar
+
+
 class Batch:
     """Device-side description of a batch of column-major SPD systems.
 
