@@ -351,6 +351,7 @@ def main():
     ap.add_argument("--nb", type=int, default=0, help="potrf panel width (0: library default)")
     ap.add_argument("--nbi", type=int, default=0, help="potrf diagonal block (0: library default)")
     ap.add_argument("--no-lookahead", action="store_true", help="single-stream factorisation")
+    ap.add_argument("--lookahead-mode", type=int, default=-1, help="look-ahead mode 0/1/2 (-1: library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -367,6 +368,8 @@ def main():
         la.set_potrf_blocking(h, args.nb, args.nbi)
     if args.no_lookahead:
         la.set_potrf_lookahead(h, False)
+    if args.lookahead_mode >= 0:
+        la.set_potrf_lookahead(h, args.lookahead_mode)
     paths, acc = {}, {}
 
     log(f"[bench] rbf inputs n={n} nrhs={nrhs}")
@@ -396,6 +399,16 @@ def main():
         log(f"[bench] rbf mixed: median {medmx:.3f} ms  {paths['rbf_mixed_fp32']['tflops']:.2f} TFLOP/s  "
             f"eta {eta:.2e}  iters {r['iters']}")
         assert max(r["iters"]) >= 0 and eta <= TOL, "mixed path did not reach the tolerance"
+        del r
+        log("[bench] rbf mixed split-bf16x3 dsposv")
+        r = run_mixed(h, A, B, args.steps, args.warmup, "bf16x3", profile=False)
+        medb = median(r["step_ms"])
+        eta = backward_error(h, A, r["X"], B)
+        paths["rbf_mixed_bf16x3"] = {"tflops": work / (medb * 1e-3) / 1e12, "ms": medb, "step_ms": r["step_ms"],
+                                     "iters": r["iters"], "fell_back_to_fp64": any(i < 0 for i in r["iters"])}
+        acc["rbf_mixed_bf16x3"] = {"backward_error": eta, "reported_berr": r["berr"]}
+        log(f"[bench] rbf mixed bf16x3: median {medb:.3f} ms  {paths['rbf_mixed_bf16x3']['tflops']:.2f} TFLOP/s  "
+            f"eta {eta:.2e}  iters {r['iters']}")
         del r
     del A, B
     torch.cuda.empty_cache()

@@ -120,7 +120,9 @@ rmh_status rmh_spotrs(rmh_handle h, int upper, size_t n, size_t nrhs, const floa
 /* Precision of the factorisation inside rmh_dsposv. */
 typedef enum rmh_factor_precision {
     RMH_FACTOR_FP32 = 0,   /* trailing updates on FP32 MFMA, FP32 accumulation */
-    RMH_FACTOR_BF16X3 = 1  /* FP32 split into three BF16 terms, BF16 MFMA, FP32 accumulation */
+    RMH_FACTOR_BF16X3 = 1  /* FP32 trailing updates with each operand split into two BF16 terms,
+                              three BF16 MFMA products (hi*hi + hi*lo + lo*hi), FP32 accumulation
+                              (lower storage; upper storage uses FP32 MFMA) */
 } rmh_factor_precision;
 
 /* Solve A X = B (A SPD, only the `upper`/lower triangle read, A unchanged)
@@ -214,9 +216,16 @@ rmh_status rmh_batched_autotune(rmh_handle h, int fp64, size_t batch, const int6
  * trailing update), nb_inner the diagonal block inside a panel (<= 64).
  * 0 restores the defaults.  Applies to later calls on this handle. */
 rmh_status rmh_set_potrf_blocking(rmh_handle h, size_t nb, size_t nb_inner);
-/* Look-ahead (on by default): rmh_?potrf factors panel k+1 on a second,
- * handle-owned stream while the caller's stream applies the rest of panel
- * k's trailing update; the call still completes in the caller's stream order. */
+/* Precision of the trailing updates of rmh_spotrf on this handle:
+ * RMH_FACTOR_FP32 (default, FP32 MFMA) or RMH_FACTOR_BF16X3 (split-BF16x3,
+ * lower storage).  rmh_dsposv sets its own choice for its factorisation. */
+rmh_status rmh_set_spotrf_precision(rmh_handle h, rmh_factor_precision prec);
+/* Look-ahead (mode 1, the default): rmh_?potrf factors panel k+1 on a
+ * second, handle-owned stream while the caller's stream applies the rest of
+ * panel k's trailing update; the call still completes in the caller's stream
+ * order.  Mode 2 also limits the trailing update to one workgroup per compute
+ * unit so that the panel kernel's workgroups run beside it; mode 0 is a
+ * single stream. */
 rmh_status rmh_set_potrf_lookahead(rmh_handle h, int enable);
 
 /* ---------------------------------------------------------- profiling -- */

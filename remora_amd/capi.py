@@ -77,6 +77,7 @@ SIGNATURES = {
     "rmh_batched_autotune": (ST, [P, I, S, P, S, P, C.POINTER(D)]),
     "rmh_set_potrf_blocking": (ST, [P, S, S]),
     "rmh_set_potrf_lookahead": (ST, [P, I]),
+    "rmh_set_spotrf_precision": (ST, [P, I]),
     "rmh_profile_enable": (ST, [P, I]),
     "rmh_profile_reset": (ST, [P]),
     "rmh_profile_read": (ST, [P, C.c_char_p, C.POINTER(D), C.POINTER(D), C.POINTER(I64)]),

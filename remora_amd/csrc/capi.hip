@@ -238,6 +238,17 @@ rmh_status trsm_impl(Context* h, bool left, bool upper, bool trans, bool unit, i
 //               on a second stream meanwhile (look-ahead)
 // A failed pivot records LAPACK's info (the first failure wins); the
 // remaining launches still run, on values that no longer matter.
+// Trailing update of potrf: the MFMA GEMM in T, or for FP32 factors the
+// split-BF16x3 kernel when the handle asks for it (lower, 128 x 128 tiles).
+template <typename T>
+static rmh_status launch_trailing(Context* h, const GemmDesc<T>& g, Tile tile, hipStream_t st) {
+    if constexpr (sizeof(T) == 4) {
+        if ((h->bf16x3_trailing || h->spotrf_bf16x3) && tile == Tile::T128x128 && g.part == Part::Lower)
+            return launch_gemm_bf16x3(g, st);
+    }
+    return launch_gemm<T>(g, tile, st);
+}
+
 template <typename T>
 rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int64_t* d_info, hipStream_t s) {
     const int64_t nb = (int64_t)h->nb, nbi = (int64_t)h->nbi;
@@ -251,6 +262,7 @@ rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
     if (n == 0) return RMH_STATUS_SUCCESS;
     const int64_t rs = upper ? lda : 1, cs = upper ? 1 : lda;
     auto V = [&](int64_t i, int64_t j) { return A + i * rs + j * cs; };
+    const bool lookahead = h->lookahead && n > 2 * nb;
 
     // Factor the panel V[k0:, k0:k0+kw] left-looking, one fused kernel per
     // block column [j0, j0+jb): it applies the updates of the panel's earlier
@@ -286,9 +298,10 @@ rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
         g.C = A + c0 + c0 * lda; g.ldc = lda;
         const double flops = 2.0 * kw * ((double)cols * (cols + 1) / 2.0 + (double)(rows - cols) * cols);
         const Tile tile = pick_tile(g.M, g.N, g.part);
+        g.one_per_cu = lookahead && h->share_cus && tile == Tile::T128x128;
         int slot;
         RMH_TRY(h->prof_begin(st, tile == Tile::T128x128 ? PROF_TRAILING : PROF_TRAILING_SMALL, flops, &slot));
-        RMH_TRY(launch_gemm<T>(g, tile, st));
+        RMH_TRY(launch_trailing<T>(h, g, tile, st));
         return h->prof_end(st, slot);
     };
 
@@ -298,7 +311,6 @@ rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
     // ranges they write are disjoint.
     hipStream_t s2 = nullptr;
     hipEvent_t ev_ready = nullptr, ev_done = nullptr;
-    const bool lookahead = h->lookahead && n > 2 * nb;
     if (lookahead) RMH_TRY(h->side_stream(&s2, &ev_ready, &ev_done));
     RMH_TRY(factor_panel(0, std::min(nb, n), s));
     for (int64_t k0 = 0; k0 < n; k0 += nb) {
@@ -681,9 +693,18 @@ rmh_status rmh_batched_autotune(rmh_handle h, int fp64, size_t batch, const int6
     return batched_autotune_impl<float>(ctx(h), (int64_t)batch, h_n, (int64_t)nrhs, best, best_ms);
 }
 
+rmh_status rmh_set_spotrf_precision(rmh_handle h, rmh_factor_precision prec) {
+    if (h == nullptr) return fail(RMH_STATUS_INVALID_HANDLE, "NULL handle");
+    RMH_REQUIRE(prec == RMH_FACTOR_FP32 || prec == RMH_FACTOR_BF16X3, "set_spotrf_precision: unknown precision");
+    ctx(h)->spotrf_bf16x3 = prec == RMH_FACTOR_BF16X3;
+    return RMH_STATUS_SUCCESS;
+}
+
 rmh_status rmh_set_potrf_lookahead(rmh_handle h, int enable) {
     if (h == nullptr) return fail(RMH_STATUS_INVALID_HANDLE, "NULL handle");
+    RMH_REQUIRE(enable >= 0 && enable <= 2, "set_potrf_lookahead: mode must be 0, 1 or 2");
     ctx(h)->lookahead = enable != 0;
+    ctx(h)->share_cus = enable == 2;
     return RMH_STATUS_SUCCESS;
 }
 

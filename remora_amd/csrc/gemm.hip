@@ -85,7 +85,7 @@ struct KParams {
 };
 
 constexpr int kThreads = 256;
-constexpr int kPad = 16;
+constexpr int kPad = 2;  // LDS row padding (elements): 16 gave heavy bank conflicts on the operand reads
 constexpr int kXcd = 8;
 constexpr int kGroupM = 8;
 constexpr int64_t kMaxLd = (int64_t)1 << 23;
@@ -346,6 +346,147 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     }
 }
 
+// ---- split-BF16x3 variant of the FP32 trailing update ------------------
+// C = alpha * A * B^T + beta * C on the lower trapezoid (A: M x K, B: N x K,
+// both column-major), every FP32 operand split as x = hi + lo with hi, lo
+// BF16 (lo the BF16 rounding of x - hi), and the product formed as
+// hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16 with FP32 accumulation:
+// about 16 significant bits per product, at three BF16 MFMAs per FP32-sized
+// block instead of eight FP32 MFMAs.  The split happens once per element, when
+// the tile is staged into LDS as [row][k] (k contiguous, so a lane's eight
+// depth values are one 16-byte read).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void gemm_bf16x3_kernel(KParams<float> p) {
+    constexpr int BM = 128, BN = 128, BK = 32, LDK = BK + 8;  // LDS row: 32 depth values + pad
+    constexpr int WM = 64, WN = 64, FM = 4, FN = 4;
+    __shared__ __bf16 Ah[2][BM * LDK], Al[2][BM * LDK];
+    __shared__ __bf16 Bh[2][BN * LDK], Bl[2][BN * LDK];
+
+    const int64_t G = gridDim.x;
+    int64_t pid = blockIdx.x;
+    {
+        const int64_t full = (G / kXcd) * kXcd;
+        if (pid < full) pid = (pid % kXcd) * (G / kXcd) + pid / kXcd;
+    }
+    int64_t tm, tn;
+    lower_trapezoid_tile(pid, p.tiles_n, tm, tn);
+    const int64_t m0 = tm * BM, n0 = tn * BN;
+    const int64_t M = p.M, N = p.N, K = p.K;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int lr = lane & 15, lk = lane >> 4;
+
+    // staging: thread t owns row t % 128 and depths 16*(t/128) .. +15 of both tiles
+    const int srow = tid % 128, sk0 = 16 * (tid / 128);
+    const int mmax = (int)min((int64_t)BM, M - m0) - 1, nmax = (int)min((int64_t)BN, N - n0) - 1;
+    const int lda32 = (int)p.lda, ldb32 = (int)p.ldb;
+    float ra[16], rb[16];
+    int st_kmax = 0;
+    auto load_tiles = [&](int64_t k0) {
+        const int kmax = (int)min((int64_t)BK, K - k0) - 1;
+        st_kmax = kmax;
+        const float* abase = p.A + m0 + k0 * p.lda;
+        const float* bbase = p.B + n0 + k0 * p.ldb;
+        const int cm = min(srow, mmax), cn = min(srow, nmax);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int ck = min(sk0 + r, kmax);
+            ra[r] = abase[cm + ck * lda32];
+            rb[r] = bbase[cn + ck * ldb32];
+        }
+    };
+    auto split_store = [&](const float (&v)[16], bool row_ok, int kmax, __bf16* hi, __bf16* lo) {
+        bf16x8 h0, h1, l0, l1;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const float x = (row_ok && sk0 + r <= kmax) ? v[r] : 0.0f;
+            const __bf16 xh = (__bf16)x;
+            const __bf16 xl = (__bf16)(x - (float)xh);
+            if (r < 8) { h0[r] = xh; l0[r] = xl; }
+            else { h1[r - 8] = xh; l1[r - 8] = xl; }
+        }
+        *reinterpret_cast<bf16x8*>(hi) = h0;
+        *reinterpret_cast<bf16x8*>(hi + 8) = h1;
+        *reinterpret_cast<bf16x8*>(lo) = l0;
+        *reinterpret_cast<bf16x8*>(lo + 8) = l1;
+    };
+    auto store_tiles = [&](int buf) {
+        const int kmax = st_kmax;
+        split_store(ra, srow <= mmax, kmax, &Ah[buf][srow * LDK + sk0], &Al[buf][srow * LDK + sk0]);
+        split_store(rb, srow <= nmax, kmax, &Bh[buf][srow * LDK + sk0], &Bl[buf][srow * LDK + sk0]);
+    };
+
+    f32x4 acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int64_t nk = (K + BK - 1) / BK;
+    if (nk > 0) {
+        load_tiles(0);
+        store_tiles(0);
+    }
+    __syncthreads();
+    for (int64_t kt = 0; kt < nk; ++kt) {
+        const int buf = (int)(kt & 1);
+        if (kt + 1 < nk) load_tiles((kt + 1) * BK);
+        bf16x8 ah[FM], al[FM], bh[FN], bl[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+            const int row = wm * WM + i * 16 + lr;
+            ah[i] = *reinterpret_cast<const bf16x8*>(&Ah[buf][row * LDK + 8 * lk]);
+            al[i] = *reinterpret_cast<const bf16x8*>(&Al[buf][row * LDK + 8 * lk]);
+        }
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int row = wn * WN + j * 16 + lr;
+            bh[j] = *reinterpret_cast<const bf16x8*>(&Bh[buf][row * LDK + 8 * lk]);
+            bl[j] = *reinterpret_cast<const bf16x8*>(&Bl[buf][row * LDK + 8 * lk]);
+        }
+        // D = B-block x A-block^T: lane items are 4 columns (n) of one row (m) of C
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
+            }
+        if (kt + 1 < nk) store_tiles(buf ^ 1);
+        __syncthreads();
+    }
+
+    const float alpha = p.alpha, beta = p.beta;
+    float* cbase = p.C + m0 + n0 * p.ldc;
+    const int ldc32 = (int)p.ldc;
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+        const int mm = wm * WM + i * 16 + lr;
+        const bool row_ok = m0 + mm < M;
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            float cold[4];
+            bool ok[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int nn = wn * WN + j * 16 + 4 * lk + r;
+                const int64_t gm = m0 + mm, gn = n0 + nn;
+                ok[r] = row_ok && gn < N && gm >= gn;
+                cold[r] = (ok[r] && beta != 0.f) ? cbase[mm + nn * ldc32] : 0.f;
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                if (!ok[r]) continue;
+                const int nn = wn * WN + j * 16 + 4 * lk + r;
+                const float v = alpha * acc[i][j][r];
+                cbase[mm + nn * ldc32] = (beta == 0.f) ? v : v + beta * cold[r];
+            }
+        }
+    }
+}
+
 // C = alpha * sum_s ws[s] + beta * C over the split-K partials.
 template <typename T>
 __global__ void split_reduce_kernel(const T* __restrict__ ws, int splits, int64_t M, int64_t N, T alpha, T beta, T* C,
@@ -389,14 +530,18 @@ rmh_status launch_cfg(const GemmDesc<T>& d, hipStream_t stream) {
     p.ntiles = ntiles;
     dim3 grid((unsigned)ntiles, (unsigned)std::max(1, d.splits), (unsigned)std::max<int64_t>(1, d.batch)),
         block(kThreads);
+    // one workgroup per compute unit: pad the LDS request past half of it
+    constexpr size_t kStatic = 2 * BK * (BM + kPad) * sizeof(T) + 2 * BK * (BN + kPad) * sizeof(T);
+    constexpr size_t kOnePerCu = 96 * 1024;
+    const size_t dyn = (d.one_per_cu && kStatic < kOnePerCu) ? kOnePerCu - kStatic : 0;
     if (!d.transA && !d.transB)
-        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, false, false, PART>), grid, block, 0, stream, p);
+        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, false, false, PART>), grid, block, dyn, stream, p);
     else if (!d.transA && d.transB)
-        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, false, true, PART>), grid, block, 0, stream, p);
+        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, false, true, PART>), grid, block, dyn, stream, p);
     else if (d.transA && !d.transB)
-        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, true, false, PART>), grid, block, 0, stream, p);
+        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, true, false, PART>), grid, block, dyn, stream, p);
     else
-        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, true, true, PART>), grid, block, 0, stream, p);
+        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, true, true, PART>), grid, block, dyn, stream, p);
     RMH_TRY(check_launch("gemm_kernel"));
     if (d.splits > 1) {
         const int64_t total = d.M * d.N;
@@ -459,6 +604,30 @@ rmh_status launch_symm(const GemmDesc<T>& d, bool upper, hipStream_t stream) {
 }
 template rmh_status launch_symm<double>(const GemmDesc<double>&, bool, hipStream_t);
 template rmh_status launch_symm<float>(const GemmDesc<float>&, bool, hipStream_t);
+
+rmh_status launch_gemm_bf16x3(const GemmDesc<float>& d, hipStream_t stream) {
+    if (d.M <= 0 || d.N <= 0) return RMH_STATUS_SUCCESS;
+    if (d.part != Part::Lower || d.transA || !d.transB || d.splits > 1 || d.batch > 1 || d.k_from_row || d.M < d.N)
+        return fail(RMH_STATUS_INVALID_VALUE, "gemm_bf16x3: only C -= A B^T on a lower trapezoid");
+    if (d.lda > kMaxLd || d.ldb > kMaxLd || d.ldc > kMaxLd)
+        return fail(RMH_STATUS_INVALID_VALUE, "gemm_bf16x3: leading dimension above 2^23");
+    KParams<float> p{};
+    p.M = d.M; p.N = d.N; p.K = d.K;
+    p.alpha = d.alpha; p.beta = d.beta;
+    p.A = d.A; p.lda = d.lda; p.B = d.B; p.ldb = d.ldb; p.C = d.C; p.ldc = d.ldc;
+    p.tiles_m = (d.M + 127) / 128;
+    p.tiles_n = (d.N + 127) / 128;
+    const int64_t tn = p.tiles_n, tmm = p.tiles_m;
+    const int64_t ntiles = tn * (tn + 1) / 2 + (tmm - tn) * tn;
+    if (ntiles <= 0) return RMH_STATUS_SUCCESS;
+    p.ntiles = ntiles;
+    p.batch_ny = 1;
+    p.k_chunk = d.K;
+    constexpr size_t kStatic = 4 * 128 * 40 * 2 * 2;  // Ah, Al, Bh, Bl, double-buffered
+    const size_t dyn = (d.one_per_cu && kStatic < 96 * 1024) ? 96 * 1024 - kStatic : 0;
+    hipLaunchKernelGGL(gemm_bf16x3_kernel, dim3((unsigned)ntiles), dim3(kThreads), dyn, stream, p);
+    return check_launch("gemm_bf16x3_kernel");
+}
 
 Tile pick_tile(int64_t M, int64_t N, Part part) {
     const int64_t t128 = ((M + 127) / 128) * ((N + 127) / 128) / (part == Part::Full ? 1 : 2);

@@ -189,7 +189,8 @@ static rmh_status residual_berr(Context* h, bool upper, int64_t n, int64_t nrhs,
 rmh_status dsposv_impl(Context* h, bool upper, rmh_factor_precision prec, int64_t n, int64_t nrhs, const double* A,
                        int64_t lda, const double* B, int64_t ldb, double* X, int64_t ldx, double tol, int max_iter,
                        int* iters, double* berr, hipStream_t s) {
-    if (prec != RMH_FACTOR_FP32) return fail(RMH_STATUS_NOT_SUPPORTED, "dsposv: factor precision not available");
+    if (prec != RMH_FACTOR_FP32 && prec != RMH_FACTOR_BF16X3)
+        return fail(RMH_STATUS_INVALID_VALUE, "dsposv: unknown factor precision");
     MixedWs w;
     RMH_TRY(mixed_ws(h, n, nrhs, &w));
     // 1. FP32 copy of the stored triangle and ||A||_inf
@@ -201,7 +202,21 @@ rmh_status dsposv_impl(Context* h, bool upper, rmh_factor_precision prec, int64_
     hipLaunchKernelGGL(max_kernel, dim3(1), dim3(256), 0, s, w.rowsum, n, w.anorm);
     RMH_TRY(check_launch("max_kernel"));
     // 2. factor in reduced precision (FP64 diagonal blocks, FP32 MFMA updates)
-    RMH_TRY(potrf_impl<float>(h, upper, n, w.A32, n, w.info, s));
+    {
+        struct Bf16x3 {  // the split-BF16x3 trailing updates for this factorisation only
+            Context* h;
+            bool saved;
+            Bf16x3(Context* c, bool on) : h(c), saved(c->spotrf_bf16x3) {
+                h->bf16x3_trailing = on;
+                h->spotrf_bf16x3 = false;  // dsposv's own choice wins
+            }
+            ~Bf16x3() {
+                h->bf16x3_trailing = false;
+                h->spotrf_bf16x3 = saved;
+            }
+        } mode(h, prec == RMH_FACTOR_BF16X3);
+        RMH_TRY(potrf_impl<float>(h, upper, n, w.A32, n, w.info, s));
+    }
     int64_t info = 0;
     RMH_TRY_HIP(hipMemcpyAsync(&info, w.info, sizeof(info), hipMemcpyDeviceToHost, s));
     RMH_TRY_HIP(hipStreamSynchronize(s));

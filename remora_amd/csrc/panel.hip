@@ -33,7 +33,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kRW = 64;  // rows per workgroup
-constexpr int kKc = 32;  // K chunk staged in LDS
+constexpr int kKc = 16;  // K chunk staged in LDS (keeps LDS at 58 KB for NB = 32)
 // LDS row strides (doubles) of the k-major chunk buffers: == 16 (mod 32), so
 // the two 16-lane halves of a 32-lane LDS pass hit different banks
 constexpr int stride16(int w) { return ((w + 15) / 32) * 32 + 16; }

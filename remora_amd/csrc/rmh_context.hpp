@@ -33,6 +33,13 @@ struct Context {
     size_t nb = kDefaultNb, nbi = kDefaultNbi;
     int prof_mask = 0;  // bit i: record launches of ProfClass i
     bool lookahead = true;
+    // look-ahead variant: trailing updates take one workgroup per compute
+    // unit so that a panel workgroup fits beside it
+    bool share_cus = false;
+    // FP32 factorisations run their 128 x 128 lower trailing updates on the
+    // split-BF16x3 kernel (set by rmh_dsposv for RMH_FACTOR_BF16X3)
+    bool bf16x3_trailing = false;
+    bool spotrf_bf16x3 = false;  // rmh_set_spotrf_precision
     hipStream_t side = nullptr;  // look-ahead stream of potrf (default priority)
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     std::vector<ProfRec> recs;

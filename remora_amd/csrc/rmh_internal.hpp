@@ -67,6 +67,10 @@ struct GemmDesc {
     // op(A) is upper triangular (op(A)(i, k) = 0 for k < i): each tile skips
     // the depth below its first row (no split-K, no batch)
     bool k_from_row = false;
+    // reserve LDS so that one workgroup of this launch occupies a compute
+    // unit's share for a 128 x 128 tile and leaves room for one workgroup of
+    // a concurrent kernel (the look-ahead panel kernel)
+    bool one_per_cu = false;
 };
 
 // Launch C = alpha*op(A)*op(B) + beta*C (restricted to `part`).  `tile`
@@ -78,6 +82,11 @@ rmh_status launch_gemm(const GemmDesc<T>& d, Tile tile, hipStream_t stream);
 // (upper = false) or upper triangle read.
 template <typename T>
 rmh_status launch_symm(const GemmDesc<T>& d, bool upper, hipStream_t stream);
+
+// C = alpha*A*B^T + beta*C on a lower trapezoid (A M x K, B N x K, FP32)
+// with each operand split into two BF16 terms and three BF16 MFMAs per
+// product (hi*hi + hi*lo + lo*hi), FP32 accumulation; 128 x 128 tiles.
+rmh_status launch_gemm_bf16x3(const GemmDesc<float>& d, hipStream_t stream);
 
 // Pick a tile shape for a problem of this size (fills 256 CUs).
 Tile pick_tile(int64_t M, int64_t N, Part part);

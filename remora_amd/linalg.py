@@ -213,7 +213,14 @@ def set_potrf_blocking(h, nb=0, nb_inner=0):
     capi.check(capi.load().rmh_set_potrf_blocking(h.value, nb, nb_inner))
 
 
+def set_spotrf_precision(h, prec="fp32"):
+    p = {"fp32": capi.FACTOR_FP32, "bf16x3": capi.FACTOR_BF16X3}[prec]
+    capi.check(capi.load().rmh_set_spotrf_precision(h.value, p))
+
+
 def set_potrf_lookahead(h, enable=True):
+    """enable: False / True, or the mode 0 (off), 1 (on), 2 (on, trailing
+    update one workgroup per compute unit)."""
     capi.check(capi.load().rmh_set_potrf_lookahead(h.value, int(enable)))
 
 

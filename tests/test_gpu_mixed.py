@@ -74,3 +74,47 @@ def test_dsposv_not_positive_definite(handle):
     with pytest.raises(capi.RemoraError) as e:
         la.dsposv(handle, dev(A), dev(np.ones((50, 1))), dev(np.zeros((50, 1))))
     assert e.value.status == capi.NOT_POSITIVE_DEFINITE
+
+
+def test_spotrf_bf16x3_trailing_accuracy(handle):
+    """The split-BF16x3 trailing update keeps about 16 bits per product:
+    the FP32 factor error grows from ~1e-7 to ~1e-5, no more.  n = 8192
+    reaches the 128 x 128 trailing tiles that use it."""
+    from remora_amd import linalg as la
+    import torch
+    n = 8192
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    G = torch.randn((n, n), generator=g, dtype=torch.float64, device="cuda")
+    A = (G @ G.t()) / n
+    del G
+    A.diagonal().add_(0.1)
+    errs = {}
+    for prec in ("fp32", "bf16x3"):
+        la.set_spotrf_precision(handle, prec)
+        L = la.colmajor(A.to(torch.float32))
+        info = la.potrf(handle, L)
+        assert int(info.item()) == 0
+        Lt = torch.tril(L).double()
+        errs[prec] = float((torch.linalg.norm(A - Lt @ Lt.t()) / torch.linalg.norm(A)).item())
+        del L, Lt
+    la.set_spotrf_precision(handle, "fp32")
+    assert errs["fp32"] < 1e-6, errs
+    assert 2 * errs["fp32"] < errs["bf16x3"] < 1e-4, errs
+
+
+@pytest.mark.parametrize("n", [1000, 4096])
+def test_dsposv_bf16x3(handle, n):
+    """BF16x3 factor + FP64 refinement reaches 1e-14 (or falls back to FP64,
+    LAPACK-style) on the RBF workload."""
+    from remora_amd import linalg as la
+    A, B = _rbf_problem(n, 16)
+    dX = dev(np.zeros((n, 16)))
+    it, berr = la.dsposv(handle, dev(A), dev(B), dX, prec="bf16x3", tol=1e-14, max_iter=30)
+    assert berr <= 1e-14
+    assert synth.backward_error(A, host(dX), B) <= 2e-14
+    if it >= 0:
+        # more corrections than the FP32 factor needs on the same problem
+        dX2 = dev(np.zeros((n, 16)))
+        it32, _ = la.dsposv(handle, dev(A), dev(B), dX2, prec="fp32")
+        assert it >= it32

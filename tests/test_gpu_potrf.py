@@ -174,10 +174,10 @@ def test_potrf_lookahead_is_bitwise_identical(handle, upper):
     """Look-ahead only reorders independent launches across two streams: the
     factor must be bit-identical to the single-stream schedule."""
     from remora_amd import linalg as la
-    n = 1700
+    n = 4500  # large enough for 128 x 128 trailing tiles (mode 2 pads their LDS)
     A = synth.gram_spd(n, seed=11)
     outs = []
-    for on in (True, False):
+    for on in (1, 2, 0):
         la.set_potrf_lookahead(handle, on)
         try:
             dA = dev(A)
@@ -186,7 +186,7 @@ def test_potrf_lookahead_is_bitwise_identical(handle, upper):
             assert int(info.item()) == 0
         finally:
             la.set_potrf_lookahead(handle, True)
-    assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
 
 
 @pytest.mark.parametrize("nbi", [16, 32, 64])
