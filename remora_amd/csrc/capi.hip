@@ -339,6 +339,13 @@ rmh_status potrs_impl(Context* h, bool upper, int64_t n, int64_t nrhs, const T* 
                       hipStream_t s) {
     if (n == 0 || nrhs == 0) return RMH_STATUS_SUCCESS;
     // lower: L y = b, L^T x = y ; upper: U^T y = b, U x = y
+    if (trsm_blocked_applies(n, nrhs)) {  // one set of block inverses for both solves
+        RMH_TRY(h->reserve(trsm_inverse_workspace<T>(n, nrhs)));
+        TrsmInverses<T> inv;
+        RMH_TRY(trsm_prepare<T>(A, lda, n, upper, false, h->ws, &inv, s));
+        RMH_TRY(trsm_apply<T>(inv, true, upper, n, nrhs, T(1), B, ldb, s));
+        return trsm_apply<T>(inv, true, !upper, n, nrhs, T(1), B, ldb, s);
+    }
     RMH_TRY(trsm_impl<T>(h, true, upper, upper, false, n, nrhs, T(1), A, lda, B, ldb, s));
     return trsm_impl<T>(h, true, upper, !upper, false, n, nrhs, T(1), A, lda, B, ldb, s);
 }

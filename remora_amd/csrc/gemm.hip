@@ -132,9 +132,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         p.B += by * p.sBy + bz * p.sBz;
         p.C += by * p.sCy + bz * p.sCz;
     }
+    int64_t kb0 = 0;  // SYM: absolute depth of this split (A is addressed absolutely)
     if (gridDim.y > 1) {
         const int64_t k0s = (int64_t)blockIdx.y * p.k_chunk;
-        p.A += TA ? k0s : k0s * p.lda;
+        if (SYM == 0) p.A += TA ? k0s : k0s * p.lda;
+        else kb0 = k0s;
         p.B += TB ? k0s * p.ldb : k0s;
         p.K = min(p.K, k0s + p.k_chunk) - k0s;
         p.C = p.split_ws + (int64_t)blockIdx.y * p.M * p.N;
@@ -209,18 +211,19 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     auto load_tiles = [&](int64_t k0) {
         const int kmax = (int)min((int64_t)BK, K - k0) - 1;  // >= 0
         st_kmax = kmax;
-        const T* abase = TA ? p.A + k0 + m0 * p.lda : p.A + m0 + k0 * p.lda;
+        const int64_t ka = k0 + kb0;  // absolute depth of A (differs from k0 only for SYM splits)
+        const T* abase = TA ? p.A + ka + m0 * p.lda : p.A + m0 + ka * p.lda;
         const T* bbase = TB ? p.B + n0 + k0 * p.ldb : p.B + k0 + n0 * p.ldb;
         // SYM: stored element (m,k) is direct when (m >= k) for lower storage
         bool all_direct = true, all_mirror = false;
         if (SYM == 1) {
-            all_direct = m0 >= k0 + BK - 1;
-            all_mirror = m0 + BM - 1 < k0;
+            all_direct = m0 >= ka + BK - 1;
+            all_mirror = m0 + BM - 1 < ka;
         } else if (SYM == 2) {
-            all_direct = m0 + BM - 1 <= k0;
-            all_mirror = m0 > k0 + BK - 1;
+            all_direct = m0 + BM - 1 <= ka;
+            all_mirror = m0 > ka + BK - 1;
         }
-        const T* mbase = p.A + k0 + m0 * p.lda;  // mirrored element (m,k) at A[k + m*lda]
+        const T* mbase = p.A + ka + m0 * p.lda;  // mirrored element (m,k) at A[k + m*lda]
         if (all_direct) {
             a_mode = 0;
 #pragma unroll
@@ -237,7 +240,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
             }
         } else {
             a_mode = 0;
-            const int64_t dk = k0 - m0;
+            const int64_t dk = ka - m0;
 #pragma unroll
             for (int r = 0; r < A_PER; ++r) {
                 const int cm = min(a_m0 + r * AM_STEP, mmax), ck = min(a_k0 + r * AK_STEP, kmax);
@@ -574,19 +577,33 @@ rmh_status launch_symm_cfg(const GemmDesc<T>& d, hipStream_t stream) {
     p.abort_flag = d.abort_flag;
     p.batch_ny = 1;
     p.sAy = p.sAz = p.sBy = p.sBz = p.sCy = p.sCz = 0;
-    p.k_chunk = d.K;
     p.k_from_row = 0;
+    int splits = 1;
+    p.k_chunk = d.K;
     p.split_ws = nullptr;
+    if (d.splits > 1 && d.split_ws != nullptr && d.K > 0) {  // split-K: partials, then one reduction
+        p.k_chunk = ((d.K + d.splits - 1) / d.splits + BK - 1) / BK * BK;
+        splits = (int)((d.K + p.k_chunk - 1) / p.k_chunk);
+        p.split_ws = d.split_ws;
+        if (splits == 1) p.k_chunk = d.K;
+    }
     const int64_t ntiles = p.tiles_m * p.tiles_n;
     if (ntiles <= 0) return RMH_STATUS_SUCCESS;
     p.ntiles = ntiles;
+    const dim3 grid((unsigned)ntiles, (unsigned)splits);
     if (d.transB)
-        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, false, true, 0, SYM>), dim3((unsigned)ntiles), dim3(kThreads), 0,
-                           stream, p);
+        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, false, true, 0, SYM>), grid, dim3(kThreads), 0, stream, p);
     else
-        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, false, false, 0, SYM>), dim3((unsigned)ntiles), dim3(kThreads), 0,
-                           stream, p);
-    return check_launch("gemm_kernel(symm)");
+        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, false, false, 0, SYM>), grid, dim3(kThreads), 0, stream, p);
+    RMH_TRY(check_launch("gemm_kernel(symm)"));
+    if (splits > 1) {
+        const int64_t total = d.M * d.N;
+        const unsigned g = (unsigned)std::min<int64_t>((total + 255) / 256, 4096);
+        hipLaunchKernelGGL(split_reduce_kernel<T>, dim3(g), dim3(256), 0, stream, d.split_ws, splits, d.M, d.N, d.alpha,
+                           d.beta, d.C, d.ldc);
+        RMH_TRY(check_launch("split_reduce_kernel"));
+    }
+    return RMH_STATUS_SUCCESS;
 }
 
 }  // namespace

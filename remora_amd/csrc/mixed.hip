@@ -141,15 +141,21 @@ struct MixedWs {
     double* anorm;
     unsigned long long* eta;
     int64_t* info;
+    void* inv_ws;  // block inverses of the FP32 factor (blocked solves), or NULL
+    double* part;  // split-K partials of the residual product
 };
+
+constexpr int kResidualSplits = 8;
 
 static rmh_status mixed_ws(Context* h, int64_t n, int64_t nrhs, MixedWs* w) {
     const size_t a = align256((size_t)n * n * sizeof(float));
     const size_t c = align256((size_t)n * nrhs * sizeof(float));
     const size_t r = align256((size_t)n * nrhs * sizeof(double));
     const size_t rs = align256((size_t)n * sizeof(double));
+    const size_t iv = trsm_blocked_applies(n, nrhs) ? align256(trsm_inverse_workspace<float>(n, nrhs)) : 0;
+    const size_t pt = align256((size_t)kResidualSplits * n * nrhs * sizeof(double));
     void* base = nullptr;
-    RMH_TRY(h->reserve_slot(WS_MIXED, a + c + r + rs + 1024, &base));
+    RMH_TRY(h->reserve_slot(WS_MIXED, a + c + r + rs + 1024 + iv + pt, &base));
     char* p = static_cast<char*>(base);
     w->A32 = reinterpret_cast<float*>(p); p += a;
     w->C32 = reinterpret_cast<float*>(p); p += c;
@@ -158,6 +164,10 @@ static rmh_status mixed_ws(Context* h, int64_t n, int64_t nrhs, MixedWs* w) {
     w->anorm = reinterpret_cast<double*>(p); p += 256;
     w->eta = reinterpret_cast<unsigned long long*>(p); p += 256;
     w->info = reinterpret_cast<int64_t*>(p);
+    p += 512;
+    w->inv_ws = iv ? static_cast<void*>(p) : nullptr;
+    p += iv;
+    w->part = reinterpret_cast<double*>(p);
     return RMH_STATUS_SUCCESS;
 }
 
@@ -173,6 +183,11 @@ static rmh_status residual_berr(Context* h, bool upper, int64_t n, int64_t nrhs,
     g.M = n; g.N = nrhs; g.K = n;
     g.alpha = -1.0; g.beta = 1.0;
     g.A = A; g.lda = lda; g.B = X; g.ldb = ldx; g.C = w.R; g.ldc = n;
+    // K = n is long and M x N = n x nrhs gives few tiles: split the depth
+    if (n >= 2048) {
+        g.splits = kResidualSplits;
+        g.split_ws = w.part;
+    }
     RMH_TRY(launch_symm<double>(g, upper, s));
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, s, w.eta);
     hipLaunchKernelGGL(berr_kernel, dim3((unsigned)nrhs), dim3(256), 0, s, w.R, n, X, ldx, B, ldb, n, w.anorm, w.eta);
@@ -224,11 +239,20 @@ rmh_status dsposv_impl(Context* h, bool upper, rmh_factor_precision prec, int64_
     int it = 0;
     bool fallback = info != 0;
     int fallback_code = -3;
+    // FP32 solves with the factor: the diagonal-block inverses are built once
+    // and shared by the initial solve and every correction
+    TrsmInverses<float> inv;
+    if (!fallback && w.inv_ws != nullptr) RMH_TRY(trsm_prepare<float>(w.A32, n, n, upper, false, w.inv_ws, &inv, s));
+    auto solve32 = [&](float* C) -> rmh_status {
+        if (w.inv_ws == nullptr) return potrs_impl<float>(h, upper, n, nrhs, w.A32, n, C, n, s);
+        RMH_TRY(trsm_apply<float>(inv, true, upper, n, nrhs, 1.0f, C, n, s));
+        return trsm_apply<float>(inv, true, !upper, n, nrhs, 1.0f, C, n, s);
+    };
     if (!fallback) {
         // 3. initial solve in FP32
         hipLaunchKernelGGL((cast2d_kernel<double, float>), dim3(grid1d(n * nrhs)), dim3(256), 0, s, B, ldb, w.C32, n,
                            n, nrhs);
-        RMH_TRY(potrs_impl<float>(h, upper, n, nrhs, w.A32, n, w.C32, n, s));
+        RMH_TRY(solve32(w.C32));
         hipLaunchKernelGGL((cast2d_kernel<float, double>), dim3(grid1d(n * nrhs)), dim3(256), 0, s, w.C32, n, X, ldx,
                            n, nrhs);
         RMH_TRY(check_launch("cast2d_kernel"));
@@ -243,7 +267,7 @@ rmh_status dsposv_impl(Context* h, bool upper, rmh_factor_precision prec, int64_
             }
             hipLaunchKernelGGL((cast2d_kernel<double, float>), dim3(grid1d(n * nrhs)), dim3(256), 0, s, w.R, n, w.C32,
                                n, n, nrhs);
-            RMH_TRY(potrs_impl<float>(h, upper, n, nrhs, w.A32, n, w.C32, n, s));
+            RMH_TRY(solve32(w.C32));
             hipLaunchKernelGGL(add_cast_kernel, dim3(grid1d(n * nrhs)), dim3(256), 0, s, w.C32, n, X, ldx, n, nrhs);
             RMH_TRY(check_launch("add_cast_kernel"));
         }

@@ -63,6 +63,27 @@ template <typename T>
 rmh_status trsm_impl(Context* h, bool left, bool upper, bool trans, bool unit, int64_t m, int64_t n, T alpha,
                      const T* A, int64_t lda, T* B, int64_t ldb, hipStream_t s);
 bool trsm_blocked_applies(int64_t na, int64_t other);
+// Inverses of the diagonal blocks of a triangular matrix S (lower storage, or
+// upper storage used as S = A^T), built once and shared by every blocked
+// solve with S or S^T (both solves of potrs, every correction of dsposv).
+template <typename T>
+struct TrsmInverses {
+    const T* A = nullptr;
+    int64_t lda = 0, na = 0, nbd = 0;
+    bool upper = false, unit = false;
+    T* D = nullptr;  // na x nbd strip of block inverses (ld na)
+    T* P = nullptr;  // split-K partials
+};
+// bytes of workspace trsm_prepare needs for order na and `other` right-hand sides
+template <typename T>
+size_t trsm_inverse_workspace(int64_t na, int64_t other);
+template <typename T>
+rmh_status trsm_prepare(const T* A, int64_t lda, int64_t na, bool upper, bool unit, void* ws, TrsmInverses<T>* inv,
+                        hipStream_t s);
+// op(S) X = alpha B (left) or X op(S) = alpha B (right), op = transpose if trans
+template <typename T>
+rmh_status trsm_apply(const TrsmInverses<T>& inv, bool left, bool trans, int64_t m, int64_t n, T alpha, T* B,
+                      int64_t ldb, hipStream_t s);
 template <typename T>
 rmh_status trsm_blocked(Context* h, bool left, bool upper, bool trans, bool unit, int64_t m, int64_t n, T alpha,
                         const T* A, int64_t lda, T* B, int64_t ldb, hipStream_t s);

@@ -35,6 +35,7 @@ struct SView {
 };
 
 int64_t choose_nbd(int64_t na) {
+    if (na > 8192) return 1024;
     if (na > 4096) return 512;
     if (na > 1024) return 256;
     return 128;
@@ -101,29 +102,64 @@ int split_count(int64_t M, int64_t N, int64_t K) {
     return (int)std::max<int64_t>(2, std::min<int64_t>(sp, 64));
 }
 
+// splits of the tall update (M x N x K, M large): enough 64 x 64 tiles to
+// give every compute unit two workgroups
+int update_splits(int64_t M, int64_t N, int64_t K) {
+    const int64_t tiles = ((M + 63) / 64) * ((N + 63) / 64);
+    if (tiles >= 512) return 1;
+    int64_t sp = (512 + tiles - 1) / tiles;
+    sp = std::min<int64_t>(sp, std::max<int64_t>(1, K / 128));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(sp, 8));
+}
+
+constexpr int kMaxInvSplit = 64, kMaxUpdSplit = 8;
+
+template <typename T>
+size_t partial_elems(int64_t na, int64_t other, int64_t nbd) {
+    return std::max((size_t)kMaxInvSplit * nbd * other, (size_t)kMaxUpdSplit * na * other);
+}
+
 }  // namespace
 
 bool trsm_blocked_applies(int64_t na, int64_t other) { return na >= 512 && other <= 1024 && other * 4 <= na; }
 
 template <typename T>
-rmh_status trsm_blocked(Context* h, bool left, bool upper, bool trans, bool unit, int64_t m, int64_t n, T alpha,
-                        const T* A, int64_t lda, T* B, int64_t ldb, hipStream_t s) {
-    const int64_t na = left ? m : n, other = left ? n : m;
+size_t trsm_inverse_workspace(int64_t na, int64_t other) {
     const int64_t nbd = choose_nbd(na);
-    const SView<T> S{A, lda, upper};
-    const bool eff_upper = upper != trans;  // op(A) = S^T
-    // workspace: D strip | W (level products) | split-K partials
     const size_t d_el = (size_t)na * nbd, w_el = (size_t)na * nbd / 2 + 64 * 64;
-    const int max_split = 64;
-    const size_t p_el = (size_t)max_split * nbd * other;
-    RMH_TRY(h->reserve((d_el + w_el + p_el) * sizeof(T) + 3 * 256));
-    T* D = static_cast<T*>(h->ws);
+    const size_t p_el = partial_elems<T>(na, other, nbd);
+    return ((d_el + 63) / 64 * 64 + (w_el + 63) / 64 * 64 + p_el) * sizeof(T) + 3 * 256;
+}
+
+template <typename T>
+rmh_status trsm_prepare(const T* A, int64_t lda, int64_t na, bool upper, bool unit, void* ws, TrsmInverses<T>* inv,
+                        hipStream_t s) {
+    const int64_t nbd = choose_nbd(na);
+    const size_t d_el = (size_t)na * nbd, w_el = (size_t)na * nbd / 2 + 64 * 64;
+    T* D = static_cast<T*>(ws);
     T* W = D + (d_el + 63) / 64 * 64;
     T* P = W + (w_el + 63) / 64 * 64;
+    inv->A = A; inv->lda = lda; inv->na = na; inv->nbd = nbd; inv->upper = upper; inv->unit = unit;
+    inv->D = D; inv->P = P;
+    const SView<T> S{A, lda, upper};
+    return build_block_inverses<T>(S, na, unit, nbd, D, W, s);
+}
+
+template <typename T>
+rmh_status trsm_apply(const TrsmInverses<T>& inv, bool left, bool trans, int64_t m, int64_t n, T alpha, T* B,
+                      int64_t ldb, hipStream_t s) {
+    const int64_t na = left ? m : n, other = left ? n : m;
+    if (na != inv.na) return fail(RMH_STATUS_INVALID_VALUE, "trsm: prepared inverses of another order");
+    const int64_t nbd = inv.nbd;
+    const SView<T> S{inv.A, inv.lda, inv.upper};
+    const bool eff_upper = inv.upper != trans;  // op(A) = S^T
+    const T* D = inv.D;
+    T* P = inv.P;
+    const size_t p_el = partial_elems<T>(na, other, nbd);
+    const int64_t lda = inv.lda;
 
     if (alpha != T(1)) RMH_TRY(launch_scale<T>(B, ldb, m, n, alpha, s));
     if (alpha == T(0)) return RMH_STATUS_SUCCESS;
-    RMH_TRY(build_block_inverses<T>(S, na, unit, nbd, D, W, s));
 
     // op(A)[r0:r1, c0:c1] as an operand: S block, or a transposed S block
     auto opA = [&](int64_t r0, int64_t c0, const T** p, bool* tr) {
@@ -154,7 +190,7 @@ rmh_status trsm_blocked(Context* h, bool left, bool upper, bool trans, bool unit
         // reduction writes C (a single m-tile cannot race either)
         g.splits = split_count(g.M, g.N, g.K);
         g.split_ws = P;
-        if ((int64_t)g.splits * g.M * g.N > (int64_t)p_el) return fail(RMH_STATUS_INVALID_VALUE, "trsm: split workspace");
+        if ((size_t)g.splits * g.M * g.N > p_el) return fail(RMH_STATUS_INVALID_VALUE, "trsm: split workspace");
         RMH_TRY(launch_gemm<T>(g, Tile::T64x64, s));
 
         // update the blocks not yet solved
@@ -177,10 +213,39 @@ rmh_status trsm_blocked(Context* h, bool left, bool upper, bool trans, bool unit
             u.ldb = lda;
             u.C = B + u0 * ldb; u.ldc = ldb;
         }
-        RMH_TRY(launch_gemm<T>(u, pick_tile(u.M, u.N, Part::Full), s));
+        const Tile ut = pick_tile(u.M, u.N, Part::Full);
+        if (ut == Tile::T64x64) {  // C is disjoint from both operands
+            u.splits = update_splits(u.M, u.N, u.K);
+            if (u.splits > 1) {
+                u.split_ws = P;
+                if ((size_t)u.splits * u.M * u.N > p_el) return fail(RMH_STATUS_INVALID_VALUE, "trsm: split workspace");
+            }
+        }
+        RMH_TRY(launch_gemm<T>(u, ut, s));
     }
     return RMH_STATUS_SUCCESS;
 }
+
+template <typename T>
+rmh_status trsm_blocked(Context* h, bool left, bool upper, bool trans, bool unit, int64_t m, int64_t n, T alpha,
+                        const T* A, int64_t lda, T* B, int64_t ldb, hipStream_t s) {
+    const int64_t na = left ? m : n, other = left ? n : m;
+    RMH_TRY(h->reserve(trsm_inverse_workspace<T>(na, other)));
+    if (alpha == T(0)) return launch_scale<T>(B, ldb, m, n, T(0), s);
+    TrsmInverses<T> inv;
+    RMH_TRY(trsm_prepare<T>(A, lda, na, upper, unit, h->ws, &inv, s));
+    return trsm_apply<T>(inv, left, trans, m, n, alpha, B, ldb, s);
+}
+
+#define RMH_TRSM_INST(T)                                                                                         \
+    template size_t trsm_inverse_workspace<T>(int64_t, int64_t);                                                  \
+    template rmh_status trsm_prepare<T>(const T*, int64_t, int64_t, bool, bool, void*, TrsmInverses<T>*,          \
+                                        hipStream_t);                                                             \
+    template rmh_status trsm_apply<T>(const TrsmInverses<T>&, bool, bool, int64_t, int64_t, T, T*, int64_t,       \
+                                      hipStream_t);
+RMH_TRSM_INST(double)
+RMH_TRSM_INST(float)
+#undef RMH_TRSM_INST
 
 template rmh_status trsm_blocked<double>(Context*, bool, bool, bool, bool, int64_t, int64_t, double, const double*,
                                          int64_t, double*, int64_t, hipStream_t);
