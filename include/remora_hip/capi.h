@@ -214,7 +214,8 @@ rmh_status rmh_batched_autotune(rmh_handle h, int fp64, size_t batch, const int6
 /* ------------------------------------------------------------ tuning -- */
 /* Block sizes of rmh_?potrf: nb is the width of a panel (the K of the
  * trailing update), nb_inner the diagonal block inside a panel (<= 64).
- * 0 restores the defaults.  Applies to later calls on this handle. */
+ * 0 restores the defaults (nb 512 for FP64, 256 for FP32; nb_inner 32).
+ * Applies to later calls on this handle. */
 rmh_status rmh_set_potrf_blocking(rmh_handle h, size_t nb, size_t nb_inner);
 /* Precision of the trailing updates of rmh_spotrf on this handle:
  * RMH_FACTOR_FP32 (default, FP32 MFMA) or RMH_FACTOR_BF16X3 (split-BF16x3,

@@ -90,10 +90,19 @@ struct Smem {
     int64_t item;
 };
 
+// The workgroup's LDS.  The per-system routines below are not inlined into
+// the persistent loop (each gets its own register allocation; the loop keeps
+// almost nothing live), and they reach the LDS through this variable so the
+// accesses stay LDS instructions.
+template <typename T, int NB>
+__shared__ Smem<T, NB> g_bsm;
+
 // Factor the n x n matrix A (column-major, ld) in place.  Returns 0 or the
 // LAPACK info (k+1 of the first failing pivot).  Uniform across the block.
 template <typename T, int NB>
-__device__ int64_t factor_one(T* A, int n, int64_t lda, Smem<T, NB>& s, const int tid) {
+__device__ __attribute__((noinline)) int64_t factor_one(T* A, int n, int64_t lda) {
+    Smem<T, NB>& s = g_bsm<T, NB>;
+    const int tid = threadIdx.x;
     typedef MfmaT<T> M;
     typedef typename M::acc_t acc_t;
     constexpr int LD = Smem<T, NB>::LD;
@@ -108,9 +117,7 @@ __device__ int64_t factor_one(T* A, int n, int64_t lda, Smem<T, NB>& s, const in
         }
         __syncthreads();
         if (tid < NB) {
-            int ln = tid;  // opaque per block column: keeps the lane masks out of the k loop
-            asm volatile("" : "+v"(ln));
-            const int bad = wave_potrf<NB, LD>(s.L, s.rdiag, ln);
+            const int bad = wave_potrf<NB, LD>(s.L, s.rdiag, tid);
             if (tid == 0) s.bad = bad;
         }
         __syncthreads();
@@ -140,47 +147,83 @@ __device__ int64_t factor_one(T* A, int n, int64_t lda, Smem<T, NB>& s, const in
             }
         }
         __syncthreads();
-        // trailing update of the lower triangle of A22 (m x m)
+        // trailing update of the lower triangle of A22 (m x m): each wave
+        // takes groups of G tiles and issues all their C loads before the
+        // MFMAs, so G tiles' worth of loads are in flight at once
+        constexpr int G = 8;
         const int tm = (m + 15) / 16, ntiles = tm * (tm + 1) / 2;
         T* A22 = A + (k + kb) + (int64_t)(k + kb) * lda;
-        auto load_tile = [&](int t, T (&c)[4], int& ti, int& tj) {
-            tri_tile(t, ti, tj);
+        for (int t0 = wave * G; t0 < ntiles; t0 += 4 * G) {
+            T c[G][4];
+            int ti[G], tj[G];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = 16 * ti + M::row(lane, r), j = 16 * tj + li;
-                c[r] = (i < m && j < m && i >= j) ? A22[i + (int64_t)j * lda] : T(0);
-            }
-        };
-        T cur[4], nxt[4];
-        int ti = 0, tj = 0, nti = 0, ntj = 0;
-        if (wave < ntiles) load_tile(wave, cur, ti, tj);
-        for (int t = wave; t < ntiles; t += 4) {
-            if (t + 4 < ntiles) load_tile(t + 4, nxt, nti, ntj);
-            acc_t acc = {T(0), T(0), T(0), T(0)};
+            for (int g = 0; g < G; ++g) {
+                tri_tile(min(t0 + g, ntiles - 1), ti[g], tj[g]);
 #pragma unroll
-            for (int q = 0; q < NB / 4; ++q) {
-                const T a = s.P[(16 * ti + li) * LD + 4 * q + lk];
-                const T b = s.P[(16 * tj + li) * LD + 4 * q + lk];
-                acc = M::op(a, b, acc);
+                for (int r = 0; r < 4; ++r) {
+                    const int i = 16 * ti[g] + M::row(lane, r), j = 16 * tj[g] + li;
+                    c[g][r] = (t0 + g < ntiles && i < m && j < m && i >= j) ? A22[i + (int64_t)j * lda] : T(0);
+                }
             }
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = 16 * ti + M::row(lane, r), j = 16 * tj + li;
-                if (i < m && j < m && i >= j) A22[i + (int64_t)j * lda] = cur[r] - acc[r];
-            }
+            for (int g = 0; g < G; ++g) {
+                if (t0 + g >= ntiles) break;
+                acc_t acc = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-            for (int r = 0; r < 4; ++r) cur[r] = nxt[r];
-            ti = nti;
-            tj = ntj;
+                for (int q = 0; q < NB / 4; ++q) {
+                    const T a = s.P[(16 * ti[g] + li) * LD + 4 * q + lk];
+                    const T b = s.P[(16 * tj[g] + li) * LD + 4 * q + lk];
+                    acc = M::op(a, b, acc);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = 16 * ti[g] + M::row(lane, r), j = 16 * tj[g] + li;
+                    if (i < m && j < m && i >= j) A22[i + (int64_t)j * lda] = c[g][r] - acc[r];
+                }
+            }
         }
         __syncthreads();
     }
     return 0;
 }
 
+// Solve L_kk Y = X (forward) or L_kk^T Y = X (backward) in place in X (LDS,
+// NB x nrhs, ld XL) with L_kk in LDS (identity-padded).  Thread t handles row
+// t % NB of right-hand side t / NB; the solved entry of each step moves to
+// the other rows of its right-hand side by a lane shuffle (the NB rows of a
+// right-hand side sit in one wavefront), so a step costs a shuffle and an FMA.
+template <int NB, int LD, int XL>
+__device__ __forceinline__ void diag_solve(const double* L, double* X, int nrhs, bool forward, int tid) {
+    constexpr int R = kThreads / NB;  // right-hand sides per pass
+    const int lane = tid & 63, row = tid % NB, base = (lane / NB) * NB;
+    for (int j0 = 0; j0 < nrhs; j0 += R) {
+        const int j = j0 + tid / NB;
+        const bool ok = j < nrhs;
+        double v = ok ? X[row * XL + j] : 0.0;
+        if (forward) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                if (row == c) v = v / L[c * LD + c];
+                const double y = __shfl(v, base + c, 64);
+                if (row > c) v = fma(-L[row * LD + c], y, v);
+            }
+        } else {
+#pragma unroll
+            for (int c = NB - 1; c >= 0; --c) {
+                if (row == c) v = v / L[c * LD + c];
+                const double x = __shfl(v, base + c, 64);
+                if (row < c) v = fma(-L[c * LD + row], x, v);
+            }
+        }
+        if (ok) X[row * XL + j] = v;
+    }
+}
+
 // Solve L L^T X = B in place (B column-major n x nrhs, ld ldb).
 template <typename T, int NB>
-__device__ void solve_one(const T* A, int n, int64_t lda, T* B, int nrhs, int64_t ldb, Smem<T, NB>& s, const int tid) {
+__device__ __attribute__((noinline)) void solve_one(const T* A, int n, int64_t lda, T* B, int nrhs, int64_t ldb) {
+    Smem<T, NB>& s = g_bsm<T, NB>;
+    const int tid = threadIdx.x;
     constexpr int LD = Smem<T, NB>::LD, XL = kMaxRhs + 1;
     auto load_diag = [&](int k, int kb) {
         for (int e = tid; e < NB * NB; e += kThreads) {
@@ -197,16 +240,12 @@ __device__ void solve_one(const T* A, int n, int64_t lda, T* B, int nrhs, int64_
         const int kb = min(NB, n - k), m = n - k - kb;
         load_diag(k, kb);
         __syncthreads();
-        if (tid < nrhs) {  // L_kk y = b, one right-hand side per thread, in LDS
-            for (int c = 0; c < kb; ++c) {
-                double sum = s.X[c * XL + tid];
-                for (int t = 0; t < c; ++t) sum = fma(-s.L[c * LD + t], s.X[t * XL + tid], sum);
-                const double y = sum / s.L[c * LD + c];
-                s.X[c * XL + tid] = y;
-                B[(k + c) + (int64_t)tid * ldb] = (T)y;
-            }
-        }
+        diag_solve<NB, LD, XL>(s.L, s.X, nrhs, true, tid);  // L_kk y = b
         __syncthreads();
+        for (int e = tid; e < kb * nrhs; e += kThreads) {
+            const int i = e % kb, j = e / kb;
+            B[(k + i) + (int64_t)j * ldb] = (T)s.X[i * XL + j];
+        }
         for (int r = tid; r < m; r += kThreads) {  // b_r -= L[r, blk] y_blk
             const T* lrow = A + (k + kb + r) + (int64_t)k * lda;
             double l[NB];
@@ -228,16 +267,12 @@ __device__ void solve_one(const T* A, int n, int64_t lda, T* B, int nrhs, int64_
         const int k = q * NB, kb = min(NB, n - k);
         load_diag(k, kb);
         __syncthreads();
-        if (tid < nrhs) {  // L_kk^T x = y, one right-hand side per thread, in LDS
-            for (int c = kb - 1; c >= 0; --c) {
-                double sum = s.X[c * XL + tid];
-                for (int t = c + 1; t < kb; ++t) sum = fma(-s.L[t * LD + c], s.X[t * XL + tid], sum);
-                const double x = sum / s.L[c * LD + c];
-                s.X[c * XL + tid] = x;
-                B[(k + c) + (int64_t)tid * ldb] = (T)x;
-            }
-        }
+        diag_solve<NB, LD, XL>(s.L, s.X, nrhs, false, tid);  // L_kk^T x = y
         __syncthreads();
+        for (int e = tid; e < kb * nrhs; e += kThreads) {
+            const int i = e % kb, j = e / kb;
+            B[(k + i) + (int64_t)j * ldb] = (T)s.X[i * XL + j];
+        }
         for (int r = tid; r < k; r += kThreads) {  // y_r -= L[blk, r]^T x_blk
             const T* lcol = A + k + (int64_t)r * lda;  // L(k + c, r), contiguous in c
             double l[NB];
@@ -257,7 +292,7 @@ __device__ void solve_one(const T* A, int n, int64_t lda, T* B, int nrhs, int64_
 
 template <typename T, int NB>
 __global__ __launch_bounds__(kThreads) void batched_posv_kernel(BArgs<T> a) {
-    __shared__ Smem<T, NB> s;
+    Smem<T, NB>& s = g_bsm<T, NB>;
     const int tid = threadIdx.x;
     for (;;) {
         if (tid == 0) s.item = (int64_t)atomicAdd(&a.counters[0], 1u);
@@ -272,13 +307,8 @@ __global__ __launch_bounds__(kThreads) void batched_posv_kernel(BArgs<T> a) {
             info = -1;
         } else if (n64 > 0) {
             const int n = (int)n64;
-            // an opaque copy of the thread index: values derived from it are
-            // recomputed per matrix instead of being hoisted out of the
-            // persistent loop (where they would hold registers throughout)
-            int t = tid;
-            asm volatile("" : "+v"(t));
-            if (a.factor) info = factor_one<T, NB>(a.A[mat], n, a.lda[mat], s, t);
-            if (info == 0 && a.nrhs > 0) solve_one<T, NB>(a.A[mat], n, a.lda[mat], a.B[mat], a.nrhs, a.ldb[mat], s, t);
+            if (a.factor) info = factor_one<T, NB>(a.A[mat], n, a.lda[mat]);
+            if (info == 0 && a.nrhs > 0) solve_one<T, NB>(a.A[mat], n, a.lda[mat], a.B[mat], a.nrhs, a.ldb[mat]);
         }
         if (tid == 0 && a.info != nullptr) a.info[mat] = info;
     }

@@ -251,7 +251,8 @@ static rmh_status launch_trailing(Context* h, const GemmDesc<T>& g, Tile tile, h
 
 template <typename T>
 rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int64_t* d_info, hipStream_t s) {
-    const int64_t nb = (int64_t)h->nb, nbi = (int64_t)h->nbi;
+    const int64_t nb = (int64_t)(h->nb != 0 ? h->nb : (sizeof(T) == 8 ? Context::kDefaultNbF64 : Context::kDefaultNbF32));
+    const int64_t nbi = (int64_t)h->nbi;
     // workspace: [0, 256) info when the caller passes none, then the side
     // buffer of the diagonal-block factors (n x nbi, ld n)
     RMH_TRY(h->reserve(256 + (size_t)std::max<int64_t>(n, 1) * nbi * sizeof(T)));
@@ -717,10 +718,11 @@ rmh_status rmh_set_potrf_lookahead(rmh_handle h, int enable) {
 
 rmh_status rmh_set_potrf_blocking(rmh_handle h, size_t nb, size_t nb_inner) {
     if (h == nullptr) return fail(RMH_STATUS_INVALID_HANDLE, "NULL handle");
-    if (nb == 0) nb = Context::kDefaultNb;
     if (nb_inner == 0) nb_inner = Context::kDefaultNbi;
     RMH_REQUIRE(nb_inner <= 64, "set_potrf_blocking: nb_inner must be <= 64");
-    RMH_REQUIRE(nb % nb_inner == 0, "set_potrf_blocking: nb must be a multiple of nb_inner");
+    RMH_REQUIRE(nb % nb_inner == 0 && Context::kDefaultNbF64 % nb_inner == 0 &&
+                    Context::kDefaultNbF32 % nb_inner == 0,
+                "set_potrf_blocking: nb (and the default widths) must be multiples of nb_inner");
     ctx(h)->nb = nb;
     ctx(h)->nbi = nb_inner;
     return RMH_STATUS_SUCCESS;

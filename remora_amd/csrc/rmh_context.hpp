@@ -22,7 +22,9 @@ struct ProfRec {
 enum WsSlot { WS_MAIN = 0, WS_MIXED = 1, WS_FACTOR = 2, WS_ADJOINT = 3, WS_COUNT = 4 };
 
 struct Context {
-    static constexpr size_t kDefaultNb = 256;
+    // default panel width by precision: FP64 trailing updates gain from
+    // K = 512 (half the C traffic), FP32 ones keep the panel chain short
+    static constexpr size_t kDefaultNbF64 = 512, kDefaultNbF32 = 256;
     static constexpr size_t kDefaultNbi = 32;
 
     int device = 0;
@@ -30,7 +32,7 @@ struct Context {
     size_t ws_bytes = 0;
     void* aux[WS_COUNT] = {nullptr, nullptr, nullptr, nullptr};
     size_t aux_bytes[WS_COUNT] = {0, 0, 0, 0};
-    size_t nb = kDefaultNb, nbi = kDefaultNbi;
+    size_t nb = 0, nbi = kDefaultNbi;  // nb 0: the default for the precision
     int prof_mask = 0;  // bit i: record launches of ProfClass i
     bool lookahead = true;
     // look-ahead variant: trailing updates take one workgroup per compute
