@@ -306,10 +306,26 @@ def median(v):
     return s[len(s) // 2]
 
 
+def measured_traffic(dtype):
+    """HBM bytes per launch of the trailing-update kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (calibrated on a device copy of
+    known size); None when the profile is absent."""
+    path = os.path.join(ROOT, "profiles", "trailing_update_traffic.json")
+    try:
+        with open(path) as f:
+            prof = json.load(f)
+        return prof["kernels"]["trail_" + dtype]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def roofline(tr, dtype, kernel):
     achieved = tr["flops"] / (tr["ms"] * 1e-3) if tr and tr["ms"] > 0 else None
     return {"bound": "mfma", "achieved": achieved / 1e12 if achieved else None, "peak": PEAK[dtype] / 1e12,
-            "unit": "TFLOP/s", "frac": achieved / PEAK[dtype] if achieved else None, "traffic": None,
+            "unit": "TFLOP/s", "frac": achieved / PEAK[dtype] if achieved else None,
+            "traffic": measured_traffic(dtype),
+            "traffic_source": "profiles/trailing_update_traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, "
+                              "calibrated on a 2 GiB device copy)",
             "kernel": kernel, "launches": tr["launches"] if tr else None,
             "avg_launch_us": tr["ms"] * 1e3 / tr["launches"] if tr and tr["launches"] else None}
 
