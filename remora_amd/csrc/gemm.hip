@@ -276,6 +276,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         }
     };
 
+    const int lr = lane & 15;   // operand row inside a 16-block
+    const int lk = lane >> 4;   // operand k inside a 4-deep step
+    // Accumulator item (i, j, r) is element (row_of(i), col_of(j, r)) of the
+    // tile: D = C^T, so a lane's items are 4 columns of C and the 16 lanes of
+    // a group hold 16 consecutive rows.
+    auto row_of = [&](int i) { return wm * WM + i * 16 + lr; };
+    auto col_of = [&](int j, int r) { return wn * WN + j * 16 + (MF::kRowInterleaved ? 4 * r + lk : 4 * lk + r); };
+    T* cbase = p.C + m0 + n0 * p.ldc;
+    const int ldc32 = (int)p.ldc;
+
     acc_t acc[FM][FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
@@ -289,8 +299,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     }
     __syncthreads();
 
-    const int lr = lane & 15;   // operand row inside a 16-block
-    const int lk = lane >> 4;   // operand k inside a 4-deep step
     for (int64_t kt = 0; kt < nk; ++kt) {
         const int buf = (int)(kt & 1);
         if (kt + 1 < nk) load_tiles((kt + 1) * BK);
@@ -313,37 +321,63 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         __syncthreads();
     }
 
-    // ---- epilogue: D = C^T, so a lane's items are 4 columns of C and the
-    // 16 lanes of a group hold 16 consecutive rows
+    // ---- epilogue: C = alpha*acc + beta*C.  The C values of a batch of
+    // accumulator rows are all loaded (clamped addresses, no branches) before
+    // any is stored, so a batch costs one memory round trip: one batch in
+    // FP32, two in FP64 (the registers the K loop used hold them).
     const T alpha = p.alpha, beta = p.beta;
-    T* cbase = p.C + m0 + n0 * p.ldc;
-    const int ldc32 = (int)p.ldc;
-    // For each 16x16 accumulator: first load the C values it needs, then
-    // store (a store between two loads would otherwise order them).
+    const bool whole = m0 + BM <= M && n0 + BN <= N &&
+                       (PART == 0 || (PART == 1 ? m0 >= n0 + BN - 1 : m0 + BM - 1 <= n0));
+    constexpr int kBatches = sizeof(T) == 8 && FM % 2 == 0 ? 2 : 1;
+    constexpr int FMB = FM / kBatches;
 #pragma unroll
-    for (int i = 0; i < FM; ++i) {
-        const int mm = wm * WM + i * 16 + lr;
-        const bool row_ok = m0 + mm < M;
+    for (int bt = 0; bt < kBatches; ++bt) {
+        T out[FMB][FN][4];
+        if (beta != T(0)) {
 #pragma unroll
-        for (int j = 0; j < FN; ++j) {
-            T cold[4];
-            bool ok[4];
+            for (int ii = 0; ii < FMB; ++ii) {
+                const int cm = min(row_of(bt * FMB + ii), mmax);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int nn = wn * WN + j * 16 + (MF::kRowInterleaved ? 4 * r + lk : 4 * lk + r);
-                const int64_t gm = m0 + mm, gn = n0 + nn;
-                bool o = row_ok && gn < N;
-                if (PART == 1) o = o && (gm >= gn);
-                if (PART == 2) o = o && (gm <= gn);
-                ok[r] = o;
-                cold[r] = (o && beta != T(0)) ? cbase[mm + nn * ldc32] : T(0);
+                for (int j = 0; j < FN; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) out[ii][j][r] = cbase[cm + min(col_of(j, r), nmax) * ldc32];
             }
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                if (!ok[r]) continue;
-                const int nn = wn * WN + j * 16 + (MF::kRowInterleaved ? 4 * r + lk : 4 * lk + r);
-                const T v = alpha * acc[i][j][r];
-                cbase[mm + nn * ldc32] = (beta == T(0)) ? v : v + beta * cold[r];
+            for (int ii = 0; ii < FMB; ++ii)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) out[ii][j][r] = alpha * acc[bt * FMB + ii][j][r] + beta * out[ii][j][r];
+        } else {
+#pragma unroll
+            for (int ii = 0; ii < FMB; ++ii)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) out[ii][j][r] = alpha * acc[bt * FMB + ii][j][r];
+        }
+        if (whole) {
+#pragma unroll
+            for (int ii = 0; ii < FMB; ++ii)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) cbase[row_of(bt * FMB + ii) + col_of(j, r) * ldc32] = out[ii][j][r];
+        } else {
+#pragma unroll
+            for (int ii = 0; ii < FMB; ++ii) {
+                const int mm = row_of(bt * FMB + ii);
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int nn = col_of(j, r);
+                        const int64_t gm = m0 + mm, gn = n0 + nn;
+                        bool ok = mm <= mmax && nn <= nmax;
+                        if (PART == 1) ok = ok && (gm >= gn);
+                        if (PART == 2) ok = ok && (gm <= gn);
+                        if (ok) cbase[mm + nn * ldc32] = out[ii][j][r];
+                    }
             }
         }
     }
@@ -421,6 +455,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         split_store(rb, srow <= nmax, kmax, &Bh[buf][srow * LDK + sk0], &Bl[buf][srow * LDK + sk0]);
     };
 
+    // item (i, j, r) of the accumulators is tile element (row_of(i), col_of(j, r))
+    auto row_of = [&](int i) { return wm * WM + i * 16 + lr; };
+    auto col_of = [&](int j, int r) { return wn * WN + j * 16 + 4 * lk + r; };
+    float* cbase = p.C + m0 + n0 * p.ldc;
+    const int ldc32 = (int)p.ldc;
     f32x4 acc[FM][FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
@@ -461,31 +500,52 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         __syncthreads();
     }
 
+    // epilogue: every C value is loaded (clamped addresses, no branches)
+    // before any is stored, so the tile costs one memory round trip
     const float alpha = p.alpha, beta = p.beta;
-    float* cbase = p.C + m0 + n0 * p.ldc;
-    const int ldc32 = (int)p.ldc;
+    const bool whole = m0 + BM <= M && n0 + BN <= N && m0 >= n0 + BN - 1;  // inside the lower trapezoid
+    float out[FM][FN][4];
+    if (beta != 0.f) {
 #pragma unroll
-    for (int i = 0; i < FM; ++i) {
-        const int mm = wm * WM + i * 16 + lr;
-        const bool row_ok = m0 + mm < M;
+        for (int i = 0; i < FM; ++i) {
+            const int cm = min(row_of(i), mmax);
 #pragma unroll
-        for (int j = 0; j < FN; ++j) {
-            float cold[4];
-            bool ok[4];
+            for (int j = 0; j < FN; ++j)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int nn = wn * WN + j * 16 + 4 * lk + r;
-                const int64_t gm = m0 + mm, gn = n0 + nn;
-                ok[r] = row_ok && gn < N && gm >= gn;
-                cold[r] = (ok[r] && beta != 0.f) ? cbase[mm + nn * ldc32] : 0.f;
-            }
+                for (int r = 0; r < 4; ++r) out[i][j][r] = cbase[cm + min(col_of(j, r), nmax) * ldc32];
+        }
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                if (!ok[r]) continue;
-                const int nn = wn * WN + j * 16 + 4 * lk + r;
-                const float v = alpha * acc[i][j][r];
-                cbase[mm + nn * ldc32] = (beta == 0.f) ? v : v + beta * cold[r];
-            }
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) out[i][j][r] = alpha * acc[i][j][r] + beta * out[i][j][r];
+    } else {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) out[i][j][r] = alpha * acc[i][j][r];
+    }
+    if (whole) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) cbase[row_of(i) + col_of(j, r) * ldc32] = out[i][j][r];
+    } else {
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+            const int mm = row_of(i);
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int nn = col_of(j, r);
+                    if (mm <= mmax && nn <= nmax && m0 + mm >= n0 + nn) cbase[mm + nn * ldc32] = out[i][j][r];
+                }
         }
     }
 }
