@@ -2,7 +2,7 @@
 // kb (<= 64) inside a panel it
 //   1. applies the updates of the panel's earlier columns (FP64 MFMA),
 //   2. factors the kb x kb diagonal block in one wavefront, and
-//   3. solves the rows below it against the factor while the factor is in LDS
+//   3. solves the rows below it against the factor
 // -- one launch per block column.  Replaces the reference HIP backend's
 // per-16-column potrf_block_kernel + cuBLAS trsm pair
 // (include/remora/kernels/hip/potrf.hpp:41-80, 113).
@@ -11,18 +11,25 @@
 // column; columns [0, K) are the panel's already factored columns, columns
 // [K, K+kb) the block column; rows [0, rows) run from the diagonal down.
 //
-// Each workgroup owns 64 rows below the diagonal block.  It streams K-chunks
-// of the diagonal rows D = V(0:kb, 0:K) and of its own rows R through LDS and
-// accumulates, on 16x16x4 FP64 MFMA tiles,
-//     own  -= R D^T      (64 x kb)       diag -= D D^T   (lower blocks)
-// Every workgroup forms the same updated diagonal block and factors it
-// itself (right-looking, lane i holds row i, pivot entries move lane to lane
-// by readlane), so no workgroup waits for another.  Workgroup 0 writes the
-// factor to a side buffer rather than over the block, which the other
-// workgroups may still be reading; the driver copies all diagonal factors
-// into place once the factorisation is done.  Rows are then solved one per
-// lane,
-//     x_c = (a_c - sum_{t<c} x_t L_ct) / L_cc.
+// Each workgroup owns 64 rows below the diagonal block, 16 per wavefront.
+// The launches form a serial chain, so the kernel is built for latency:
+//  * Update.  Each wavefront loads its MFMA operands straight from global
+//    memory (L2) into registers -- its own 16 rows R and the diagonal rows
+//    D = V(0:kb, 0:K), 16 depths per chunk, several chunks in flight -- and
+//    accumulates own -= R D^T and its share of the lower blocks of
+//    diag -= D D^T on 16x16x4 FP64 MFMA tiles, with no LDS staging and no
+//    barriers in the K loop.
+//  * Factor and solve.  For kb <= 32 every factoring wavefront holds the
+//    updated diagonal block in lanes 0..kb-1 (lane i row i) and 64-kb own
+//    rows in the other lanes, and runs the right-looking elimination of the
+//    block on all of them (wave_chol_rows): the block lanes end with the
+//    factor L, the row lanes with x, x L^T = a.  Every wavefront and
+//    workgroup forms the same factor, so none waits for another.  (Wider
+//    blocks factor in wavefront 0 and solve the rows afterwards, four lanes
+//    per row.)  Workgroup 0 writes the factor to a side buffer rather than
+//    over the block, which the other workgroups may still be reading; the
+//    driver copies all diagonal factors into place once the factorisation
+//    is done.
 // The arithmetic is FP64 whatever the storage type (the mixed-precision
 // factorisation keeps its diagonal blocks in FP64).
 #include "rmh_internal.hpp"
@@ -33,10 +40,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kRW = 64;  // rows per workgroup
-constexpr int kKc = 16;  // K chunk staged in LDS (keeps LDS at 58 KB for NB = 32)
-// LDS row strides (doubles) of the k-major chunk buffers: == 16 (mod 32), so
-// the two 16-lane halves of a 32-lane LDS pass hit different banks
-constexpr int stride16(int w) { return ((w + 15) / 32) * 32 + 16; }
+constexpr int kKc = 16;  // K chunk: four 16x16x4 MFMA steps
 
 typedef double dx4 __attribute__((ext_vector_type(4)));
 
@@ -47,6 +51,17 @@ __device__ __forceinline__ void lower_block(int d, int& bi, int& bj) {
     bj = d;
 }
 
+// v of lane (lane ^ 1) and of lane (lane ^ 2) within each quad of lanes
+template <int CTRL>
+__device__ __forceinline__ double quad_perm(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+constexpr int kQuadXor1 = 0xB1;  // quad_perm [1,0,3,2]
+constexpr int kQuadXor2 = 0x4E;  // quad_perm [2,3,0,1]
+
 template <typename T, int NB>
 __global__ __launch_bounds__(kThreads) void potrf_panel_kernel(T* P, int64_t rs, int64_t cs, int64_t rows, int K,
                                                                int kb, int64_t* info, int64_t info_base, T* dfac,
@@ -54,19 +69,16 @@ __global__ __launch_bounds__(kThreads) void potrf_panel_kernel(T* P, int64_t rs,
     constexpr int NCB = NB / 16;              // 16-column blocks
     constexpr int NDB = NCB * (NCB + 1) / 2;  // lower blocks of the diagonal block
     constexpr int DPW = (NDB + 3) / 4;        // diagonal blocks per wave
-    constexpr int SD = stride16(NB), SR = stride16(kRW);
     constexpr int LD = NB + 1;
-    constexpr int PD = NB * kKc / kThreads, PR = kRW * kKc / kThreads;  // chunk elements per thread
-    constexpr int CD = NB * NB / kThreads, CR = kRW * NB / kThreads;    // block-column elements per thread
-    static_assert(NB % 16 == 0 && PD >= 1 && CD >= 1, "unsupported block width");
+    constexpr int CD = NB * NB / kThreads, CR = kRW * NB / kThreads;  // block-column elements per thread
+    constexpr int DEP = NB <= 32 ? 4 : 2;     // K chunks in flight per wavefront
+    constexpr int NQ = NB / 4;                // solve: columns per lane
+    static_assert(NB % 16 == 0 && CD >= 1, "unsupported block width");
 
-    // K chunks are double-buffered in LDS; two register sets keep the global
-    // loads two chunks ahead of the MFMAs
-    __shared__ double Dk[2][kKc * SD];  // Dk[b][k*SD + i]: diagonal rows, K chunk
-    __shared__ double Rk[2][kKc * SR];  // Rk[b][k*SR + i]: own rows, K chunk
-    __shared__ double L[NB * LD];       // updated diagonal block, then its factor
-    __shared__ double X[kRW * LD];      // updated own rows
+    __shared__ double L[NB * LD];  // updated diagonal block, then its factor
+    __shared__ double X[kRW * LD]; // updated own rows
     __shared__ double rdiag[NB];
+    __shared__ __align__(16) double col[4 * NB];  // pivot-column buffer per wavefront
     __shared__ int failed;
 
     const int tid = threadIdx.x, lane = tid & 63;
@@ -75,127 +87,115 @@ __global__ __launch_bounds__(kThreads) void potrf_panel_kernel(T* P, int64_t rs,
     const int nown = (int)max((int64_t)0, min((int64_t)kRW, rows - r0));
     const int nch = (K + kKc - 1) / kKc;
 
-    // block column: own rows and the diagonal block (padded with the
-    // identity), loaded first
-    double cur_r[CR], cur_d[CD];
+    // K-loop operands in the MFMA lane layout: lane l holds row l % 16 at
+    // depth l / 16 of a 4-deep step.  Loads keep the raw values (converting
+    // or masking them at load time would wait for them there).  Rows outside
+    // the matrix read row 0 and only produce values that are never stored
+    // (own rows past nown, columns of X past kb, entries of L outside the
+    // kb x kb block, which is not updated); depths past K read depth K-1 and
+    // are zeroed in A when the chunk is computed.
+    const int li = lane & 15, lk = lane >> 4;
+    const bool a_ok = 16 * wave + li < nown;
+    const T* pa = P + (a_ok ? r0 + 16 * wave + li : 0) * rs;
+    const T* pb[NCB];
+#pragma unroll
+    for (int b = 0; b < NCB; ++b) pb[b] = P + (int64_t)(16 * b + li < kb ? 16 * b + li : 0) * rs;
+    // Chunk loads are unconditional (chunks past the end read depth kmax and
+    // are never computed), so the compiler counts the loads in flight
+    // exactly and waits only for the ones it needs.
+    const int kmax = max(K - 1, 0);
+    T av[DEP][4], bv[DEP][4][NCB];
+    auto load_chunk = [&](int ch, T(&a)[4], T(&bm)[4][NCB]) {
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+            const int64_t off = (int64_t)min(ch * kKc + 4 * s4 + lk, kmax) * cs;
+            a[s4] = pa[off];
+#pragma unroll
+            for (int b = 0; b < NCB; ++b) bm[s4][b] = pb[b][off];
+        }
+    };
+#pragma unroll
+    for (int s = 0; s < DEP - 1; ++s) load_chunk(s, av[s], bv[s]);
+
+    // block column: own rows and the diagonal block, loaded after the first
+    // K chunks from always-valid (clamped) addresses; the raw values go to
+    // LDS after the K loop, masked, the diagonal block padded with the
+    // identity
+    T cur_r[CR], cur_d[CD];
 #pragma unroll
     for (int q = 0; q < CR; ++q) {
         const int e = tid + q * kThreads, i = e % kRW, c = e / kRW;
-        cur_r[q] = (i < nown && c < kb) ? (double)P[(r0 + i) * rs + (int64_t)(K + c) * cs] : 0.0;
+        cur_r[q] = P[(i < nown ? r0 + i : 0) * rs + (int64_t)(K + (c < kb ? c : 0)) * cs];
     }
 #pragma unroll
     for (int q = 0; q < CD; ++q) {
         const int e = tid + q * kThreads, i = e % NB, c = e / NB;
-        cur_d[q] = (i < kb && c <= i) ? (double)P[i * rs + (int64_t)(K + c) * cs] : (i == c ? 1.0 : 0.0);
+        const bool ok = i < kb && c <= i;
+        cur_d[q] = P[(ok ? i : 0) * rs + (int64_t)(K + (ok ? c : 0)) * cs];
     }
-    double pa_d[PD], pa_r[PR], pb_d[PD], pb_r[PR];
-    auto load_chunk = [&](int ch, double(&pd)[PD], double(&pr)[PR]) {
-        const int k0 = ch * kKc, kc = min(kKc, K - k0);
-#pragma unroll
-        for (int q = 0; q < PD; ++q) {
-            const int e = tid + q * kThreads, i = e % NB, c = e / NB;
-            pd[q] = (i < kb && c < kc) ? (double)P[i * rs + (int64_t)(k0 + c) * cs] : 0.0;
-        }
-#pragma unroll
-        for (int q = 0; q < PR; ++q) {
-            const int e = tid + q * kThreads, i = e % kRW, c = e / kRW;
-            pr[q] = (i < nown && c < kc) ? (double)P[(r0 + i) * rs + (int64_t)(k0 + c) * cs] : 0.0;
-        }
-    };
-    auto store_chunk = [&](int b, const double(&pd)[PD], const double(&pr)[PR]) {
-#pragma unroll
-        for (int q = 0; q < PD; ++q) {
-            const int e = tid + q * kThreads;
-            Dk[b][(e / NB) * SD + e % NB] = pd[q];
-        }
-#pragma unroll
-        for (int q = 0; q < PR; ++q) {
-            const int e = tid + q * kThreads;
-            Rk[b][(e / kRW) * SR + e % kRW] = pr[q];
-        }
-    };
-    if (nch > 0) load_chunk(0, pa_d, pa_r);
-    if (nch > 1) load_chunk(1, pb_d, pb_r);
 
-#pragma unroll
-    for (int q = 0; q < CR; ++q) {
-        const int e = tid + q * kThreads;
-        X[(e % kRW) * LD + e / kRW] = cur_r[q];
-    }
-#pragma unroll
-    for (int q = 0; q < CD; ++q) {
-        const int e = tid + q * kThreads;
-        L[(e % NB) * LD + e / NB] = cur_d[q];
-    }
-    if (nch > 0) store_chunk(0, pa_d, pa_r);
-    if (nch > 2) load_chunk(2, pa_d, pa_r);
     if (tid == 0) failed = -1;
-    __syncthreads();
 
-    // 1. left-looking update on MFMA: wave w owns rows [16w, 16w+16) of X
-    // (all column blocks) and diagonal blocks w, w+4, ...
+    // 1. left-looking update: wave w owns rows [16w, 16w+16) of X (all
+    // column blocks) and diagonal blocks w, w+4, ...
     dx4 acc[NCB], dacc[DPW];
 #pragma unroll
     for (int b = 0; b < NCB; ++b) acc[b] = dx4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int b = 0; b < DPW; ++b) dacc[b] = dx4{0.0, 0.0, 0.0, 0.0};
-    const int li = lane & 15, lk = lane >> 4;
-    // this wave's diagonal blocks (wave-uniform)
     int dbi[DPW], dbj[DPW];
 #pragma unroll
     for (int q = 0; q < DPW; ++q) {
         dbi[q] = dbj[q] = 0;
         if (wave + 4 * q < NDB) lower_block(wave + 4 * q, dbi[q], dbj[q]);
     }
-    // all LDS operands of a chunk are read before its MFMAs are issued; the
-    // diagonal blocks' operands are column blocks of the same reads
-    auto compute = [&](int buf) {
-        const double* dk = Dk[buf];
-        const double* rk = Rk[buf];
-        double av[kKc / 4], bv[kKc / 4][NCB];
+    auto compute = [&](int ch, const T(&a)[4], const T(&bm)[4][NCB]) {
 #pragma unroll
-        for (int s4 = 0; s4 < kKc / 4; ++s4) {
-            const int k = 4 * s4 + lk;
-            av[s4] = rk[k * SR + 16 * wave + li];
+        for (int s4 = 0; s4 < 4; ++s4) {
+            const double av4 = ch * kKc + 4 * s4 + lk < K ? (double)a[s4] : 0.0;
+            double bv4[NCB];
 #pragma unroll
-            for (int b = 0; b < NCB; ++b) bv[s4][b] = dk[k * SD + 16 * b + li];
-        }
+            for (int b = 0; b < NCB; ++b) bv4[b] = (double)bm[s4][b];
 #pragma unroll
-        for (int s4 = 0; s4 < kKc / 4; ++s4) {
-#pragma unroll
-            for (int b = 0; b < NCB; ++b)
-                acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s4], bv[s4][b], acc[b], 0, 0, 0);
+            for (int b = 0; b < NCB; ++b) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av4, bv4[b], acc[b], 0, 0, 0);
 #pragma unroll
             for (int q = 0; q < DPW; ++q) {
                 if (wave + 4 * q < NDB) {
-                    double x = bv[s4][0], y = bv[s4][0];
+                    // the zeroed depth of av4 is not in this product: mask x
+                    double x = bv4[0], y = bv4[0];
 #pragma unroll
                     for (int b = 1; b < NCB; ++b) {
-                        x = (dbi[q] == b) ? bv[s4][b] : x;
-                        y = (dbj[q] == b) ? bv[s4][b] : y;
+                        x = (dbi[q] == b) ? bv4[b] : x;
+                        y = (dbj[q] == b) ? bv4[b] : y;
                     }
+                    x = ch * kKc + 4 * s4 + lk < K ? x : 0.0;
                     dacc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, dacc[q], 0, 0, 0);
                 }
             }
         }
     };
-    // chunk c sits in LDS buffer c % 2; registers a hold chunks 0, 2, 4, ...
-    // and b chunks 1, 3, 5, ... until they are stored
-    for (int ch = 0; ch < nch; ch += 2) {
-        compute(0);
-        if (ch + 1 < nch) {
-            store_chunk(1, pb_d, pb_r);  // buffer 1 was last read before the previous barrier
-            if (ch + 3 < nch) load_chunk(ch + 3, pb_d, pb_r);
+    // chunk c sits in register slot c % DEP; the slot refilled at each step
+    // is the one computed at the step before
+    for (int ch0 = 0; ch0 < nch; ch0 += DEP) {
+#pragma unroll
+        for (int s = 0; s < DEP; ++s) {
+            const int ch = ch0 + s;
+            if (ch >= nch) break;
+            load_chunk(ch + DEP - 1, av[(s + DEP - 1) % DEP], bv[(s + DEP - 1) % DEP]);
+            compute(ch, av[s], bv[s]);
         }
-        __syncthreads();
-        if (ch + 1 >= nch) break;
-        compute(1);
-        if (ch + 2 < nch) {
-            store_chunk(0, pa_d, pa_r);
-            if (ch + 4 < nch) load_chunk(ch + 4, pa_d, pa_r);
-        }
-        __syncthreads();
     }
+#pragma unroll
+    for (int q = 0; q < CR; ++q) {
+        const int e = tid + q * kThreads, i = e % kRW, c = e / kRW;
+        X[i * LD + c] = (i < nown && c < kb) ? (double)cur_r[q] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < CD; ++q) {
+        const int e = tid + q * kThreads, i = e % NB, c = e / NB;
+        L[i * LD + c] = (i < kb && c <= i) ? (double)cur_d[q] : (i == c ? 1.0 : 0.0);
+    }
+    __syncthreads();  // the block column is in X and L
     // subtract: lane l, item r of a 16x16 tile is element (4r + l/16, l%16)
     if (K > 0) {
 #pragma unroll
@@ -208,56 +208,99 @@ __global__ __launch_bounds__(kThreads) void potrf_panel_kernel(T* P, int64_t rs,
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int i = 16 * dbi[q] + 4 * r + lk, c = 16 * dbj[q] + li;
-                    if (c <= i) L[i * LD + c] -= dacc[q][r];
+                    if (c <= i && i < kb) L[i * LD + c] -= dacc[q][r];  // the identity padding stays
                 }
             }
         }
     }
     __syncthreads();
 
-    // 2. factor the (identity-padded) diagonal block in wavefront 0
-    if (tid < NB) {
-        const int bad = wave_potrf<NB, LD>(L, rdiag, tid);
-        if (tid == 0) failed = bad;
-    }
-    __syncthreads();
-    const int bad = failed;
-    if (bad >= 0) {
-        // keep the first failure: later launches run on garbage but must not overwrite it
-        if (blockIdx.x == 0 && tid == 0)
-            atomicCAS(reinterpret_cast<unsigned long long*>(info), 0ull, (unsigned long long)(info_base + bad + 1));
-        return;
-    }
-    // the factor of the diagonal block goes to the side buffer: every
-    // workgroup of this launch still reads the block's original values
-    if (blockIdx.x == 0) {
-        for (int e = tid; e < kb * kb; e += kThreads) {
-            const int i = e % kb, c = e / kb;
-            if (c <= i) dfac[i + c * ldf] = (T)L[i * LD + c];
+    if constexpr (NB <= 32) {
+        // 2+3. wavefronts 0.. hold the diagonal block in lanes 0..NB-1 and
+        // 64-NB own rows in the other lanes, and factor the block while
+        // solving the rows along (wave_chol_rows).  Every such wavefront
+        // forms the same factor, so none waits for another.
+        constexpr int OPW = 64 - NB;  // own rows per factoring wavefront
+        constexpr int NFW = (kRW + OPW - 1) / OPW;
+        if (wave >= NFW) return;
+        const int orow = wave * OPW + lane - NB;  // own row of a lane >= NB
+        const bool is_diag = lane < NB, is_row = !is_diag && orow < kRW;
+        const double* src = is_diag ? &L[lane * LD] : &X[(is_row ? orow : 0) * LD];
+        double r[NB];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) r[c] = src[c];
+        const int bad = wave_chol_rows<NB>(r, &col[wave * NB], lane);
+        if (bad >= 0) {
+            // keep the first failure: later launches run on garbage but must not overwrite it
+            if (blockIdx.x == 0 && tid == 0)
+                atomicCAS(reinterpret_cast<unsigned long long*>(info), 0ull, (unsigned long long)(info_base + bad + 1));
+            return;
         }
-    }
-
-    // 3. own rows: X L^T = A21, one row per lane of wavefront 0 (the padded
-    // columns come out zero and are not stored)
-    if (tid >= nown) return;
-    double x[NB];
+        if (is_row && orow < nown) {
+            const int64_t i = r0 + orow;
 #pragma unroll
-    for (int c = 0; c < NB; ++c) x[c] = X[tid * LD + c];
+            for (int c = 0; c < NB; ++c)
+                if (c < kb) P[i * rs + (int64_t)(K + c) * cs] = (T)r[c];
+        }
+        // the factor of the diagonal block goes to the side buffer: every
+        // workgroup of this launch still reads the block's original values
+        if (blockIdx.x == 0 && wave == 0 && lane < kb) {
 #pragma unroll
-    for (int c = 0; c < NB; ++c) {
-        double sum[4] = {x[c], 0.0, 0.0, 0.0};  // four independent chains
-#pragma unroll
-        for (int t = 0; t < c; ++t) sum[t & 3] = fma(-x[t], L[c * LD + t], sum[t & 3]);
-        x[c] = ((sum[0] + sum[1]) + (sum[2] + sum[3])) * rdiag[c];
-    }
-    const int64_t i = r0 + tid;
-    if (kb == NB) {
-#pragma unroll
-        for (int c = 0; c < NB; ++c) P[i * rs + (int64_t)(K + c) * cs] = (T)x[c];
+            for (int c = 0; c < NB; ++c)
+                if (c <= lane) dfac[lane + c * ldf] = (T)r[c];
+        }
     } else {
+        // 2. factor the (identity-padded) diagonal block in wavefront 0
+        if (tid < NB) {
+            const int bad = wave_potrf_lds<NB, LD>(L, rdiag, col, tid);
+            if (tid == 0) failed = bad;
+        }
+        __syncthreads();
+        const int bad = failed;
+        if (bad >= 0) {
+            // keep the first failure: later launches run on garbage but must not overwrite it
+            if (blockIdx.x == 0 && tid == 0)
+                atomicCAS(reinterpret_cast<unsigned long long*>(info), 0ull, (unsigned long long)(info_base + bad + 1));
+            return;
+        }
+        // the factor of the diagonal block goes to the side buffer: every
+        // workgroup of this launch still reads the block's original values
+        if (blockIdx.x == 0) {
+            for (int e = tid; e < kb * kb; e += kThreads) {
+                const int i = e % kb, c = e / kb;
+                if (c <= i) dfac[i + c * ldf] = (T)L[i * LD + c];
+            }
+        }
+
+        // 3. own rows: X L^T = A21.  Wave w solves rows [16w, 16w+16), lanes
+        // 4p..4p+3 row p; lane q of a quad holds columns 4s + q.  The padded
+        // columns come out zero and are not stored.
+        const int srow = 16 * wave + (lane >> 2), q = lane & 3;
+        double xs[NQ];
 #pragma unroll
-        for (int c = 0; c < NB; ++c)
-            if (c < kb) P[i * rs + (int64_t)(K + c) * cs] = (T)x[c];
+        for (int s = 0; s < NQ; ++s) xs[s] = X[srow * LD + 4 * s + q];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            const int sc = c / 4, oc = c % 4;
+            const double* lrow = &L[c * LD + q];
+            double p0 = 0.0, p1 = 0.0;  // two chains
+#pragma unroll
+            for (int s = 0; s < sc; ++s) {
+                if (s & 1) p1 = fma(xs[s], lrow[4 * s], p1);
+                else p0 = fma(xs[s], lrow[4 * s], p0);
+            }
+            if (oc > 0) p1 = (q < oc) ? fma(xs[sc], lrow[4 * sc], p1) : p1;  // columns 4sc + q < c
+            double part = p0 + p1;
+            part += quad_perm<kQuadXor1>(part);
+            part += quad_perm<kQuadXor2>(part);
+            xs[sc] = (q == oc) ? (xs[sc] - part) * rdiag[c] : xs[sc];
+        }
+        if (srow < nown) {
+            const int64_t i = r0 + srow;
+#pragma unroll
+            for (int s = 0; s < NQ; ++s)
+                if (4 * s + q < kb) P[i * rs + (int64_t)(K + 4 * s + q) * cs] = (T)xs[s];
+        }
     }
 }
 
