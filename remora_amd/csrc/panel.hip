@@ -32,13 +32,19 @@
 //    is done.
 // The arithmetic is FP64 whatever the storage type (the mixed-precision
 // factorisation keeps its diagonal blocks in FP64).
+#include <type_traits>
+
 #include "rmh_internal.hpp"
 #include "wave_chol.hpp"
 
 namespace rmh {
 namespace {
 
-constexpr int kThreads = 256;
+// Threads per workgroup: blocks up to 32 wide run eight wavefronts, two per
+// SIMD, the wavefronts w and w+4 taking alternate K chunks of the same rows
+// (one f64 MFMA chain per SIMD leaves the matrix unit half idle); 64-wide
+// blocks need the registers of four.
+constexpr int panel_threads(int nb) { return nb <= 32 ? 512 : 256; }
 constexpr int kRW = 64;  // rows per workgroup
 constexpr int kKc = 16;  // K chunk: four 16x16x4 MFMA steps
 
@@ -63,17 +69,19 @@ constexpr int kQuadXor1 = 0xB1;  // quad_perm [1,0,3,2]
 constexpr int kQuadXor2 = 0x4E;  // quad_perm [2,3,0,1]
 
 template <typename T, int NB>
-__global__ __launch_bounds__(kThreads) void potrf_panel_kernel(T* P, int64_t rs, int64_t cs, int64_t rows, int K,
+__global__ __launch_bounds__(panel_threads(NB)) void potrf_panel_kernel(T* P, int64_t rs, int64_t cs, int64_t rows, int K,
                                                                int kb, int64_t* info, int64_t info_base, T* dfac,
                                                                int64_t ldf) {
+    constexpr int kThreads = panel_threads(NB);
+    constexpr int KG = kThreads / 256;        // wavefronts sharing a row block (K groups)
     constexpr int NCB = NB / 16;              // 16-column blocks
     constexpr int NDB = NCB * (NCB + 1) / 2;  // lower blocks of the diagonal block
     constexpr int DPW = (NDB + 3) / 4;        // diagonal blocks per wave
     constexpr int LD = NB + 1;
-    constexpr int CD = NB * NB / kThreads, CR = kRW * NB / kThreads;  // block-column elements per thread
-    constexpr int DEP = NB <= 32 ? 4 : 2;     // K chunks in flight per wavefront
+    constexpr int CD = (NB * NB + kThreads - 1) / kThreads, CR = kRW * NB / kThreads;  // block-column elements per thread
+    constexpr int DEP = 2;                    // K chunks in flight per wavefront (deeper only grows the code)
     constexpr int NQ = NB / 4;                // solve: columns per lane
-    static_assert(NB % 16 == 0 && CD >= 1, "unsupported block width");
+    static_assert(NB % 16 == 0 && CR * kThreads == kRW * NB, "unsupported block width");
 
     __shared__ double L[NB * LD];  // updated diagonal block, then its factor
     __shared__ double X[kRW * LD]; // updated own rows
@@ -83,6 +91,7 @@ __global__ __launch_bounds__(kThreads) void potrf_panel_kernel(T* P, int64_t rs,
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wr = wave & 3, kg = wave >> 2;  // row block, K group (chunks kg, kg + KG, ...)
     const int64_t r0 = kb + (int64_t)blockIdx.x * kRW;  // first own row
     const int nown = (int)max((int64_t)0, min((int64_t)kRW, rows - r0));
     const int nch = (K + kKc - 1) / kKc;
@@ -95,11 +104,31 @@ __global__ __launch_bounds__(kThreads) void potrf_panel_kernel(T* P, int64_t rs,
     // kb x kb block, which is not updated); depths past K read depth K-1 and
     // are zeroed in A when the chunk is computed.
     const int li = lane & 15, lk = lane >> 4;
-    const bool a_ok = 16 * wave + li < nown;
-    const T* pa = P + (a_ok ? r0 + 16 * wave + li : 0) * rs;
+    const bool a_ok = 16 * wr + li < nown;
+    const T* pa = P + (a_ok ? r0 + 16 * wr + li : 0) * rs;
+    // This wavefront's diagonal blocks of D D^T (its share of the lower
+    // block triangle) and the order in which it holds D's column blocks:
+    // for two column blocks, waves 1 and 2 (blocks (1,0) and (1,1)) hold
+    // block 1 first, so every diagonal product reads fixed registers -- x
+    // from slot 0, y from slot 0 or 1 (diag mode dm 1 or 2; 0: none; 3:
+    // wider blocks select per product).
+    int dbi[DPW], dbj[DPW];
+#pragma unroll
+    for (int q = 0; q < DPW; ++q) {
+        dbi[q] = dbj[q] = 0;
+        if (wr + 4 * q < NDB) lower_block(wr + 4 * q, dbi[q], dbj[q]);
+    }
+    int pblk[NCB];
+#pragma unroll
+    for (int b = 0; b < NCB; ++b) pblk[b] = b;
+    int dm = 3;
+    if (NCB <= 2) {
+        if (NCB == 2 && (wr == 1 || wr == 2)) { pblk[0] = 1; pblk[NCB - 1] = 0; }
+        dm = wr >= NDB ? 0 : (wr == 1 ? 2 : 1);
+    }
     const T* pb[NCB];
 #pragma unroll
-    for (int b = 0; b < NCB; ++b) pb[b] = P + (int64_t)(16 * b + li < kb ? 16 * b + li : 0) * rs;
+    for (int b = 0; b < NCB; ++b) pb[b] = P + (int64_t)(16 * pblk[b] + li < kb ? 16 * pblk[b] + li : 0) * rs;
     // Chunk loads are unconditional (chunks past the end read depth kmax and
     // are never computed), so the compiler counts the loads in flight
     // exactly and waits only for the ones it needs.
@@ -115,7 +144,7 @@ __global__ __launch_bounds__(kThreads) void potrf_panel_kernel(T* P, int64_t rs,
         }
     };
 #pragma unroll
-    for (int s = 0; s < DEP - 1; ++s) load_chunk(s, av[s], bv[s]);
+    for (int s = 0; s < DEP - 1; ++s) load_chunk(kg + KG * s, av[s], bv[s]);
 
     // block column: own rows and the diagonal block, loaded after the first
     // K chunks from always-valid (clamped) addresses; the raw values go to
@@ -129,7 +158,7 @@ __global__ __launch_bounds__(kThreads) void potrf_panel_kernel(T* P, int64_t rs,
     }
 #pragma unroll
     for (int q = 0; q < CD; ++q) {
-        const int e = tid + q * kThreads, i = e % NB, c = e / NB;
+        const int e = min(tid + q * kThreads, NB * NB - 1), i = e % NB, c = e / NB;
         const bool ok = i < kb && c <= i;
         cur_d[q] = P[(ok ? i : 0) * rs + (int64_t)(K + (ok ? c : 0)) * cs];
     }
@@ -143,48 +172,60 @@ __global__ __launch_bounds__(kThreads) void potrf_panel_kernel(T* P, int64_t rs,
     for (int b = 0; b < NCB; ++b) acc[b] = dx4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int b = 0; b < DPW; ++b) dacc[b] = dx4{0.0, 0.0, 0.0, 0.0};
-    int dbi[DPW], dbj[DPW];
-#pragma unroll
-    for (int q = 0; q < DPW; ++q) {
-        dbi[q] = dbj[q] = 0;
-        if (wave + 4 * q < NDB) lower_block(wave + 4 * q, dbi[q], dbj[q]);
-    }
-    auto compute = [&](int ch, const T(&a)[4], const T(&bm)[4][NCB]) {
+    // One chunk: MASK zeroes the depths past K (last chunk only), DM is the
+    // diagonal mode above.  Operands go straight from the load registers to
+    // the MFMAs.
+    auto compute = [&](auto mask_c, auto dm_c, int ch, const T(&a)[4], const T(&bm)[4][NCB]) {
+        constexpr bool MASK = decltype(mask_c)::value;
+        constexpr int DM = decltype(dm_c)::value;
 #pragma unroll
         for (int s4 = 0; s4 < 4; ++s4) {
-            const double av4 = ch * kKc + 4 * s4 + lk < K ? (double)a[s4] : 0.0;
+            const bool k_ok = !MASK || ch * kKc + 4 * s4 + lk < K;
+            const double av4 = k_ok ? (double)a[s4] : 0.0;
             double bv4[NCB];
 #pragma unroll
             for (int b = 0; b < NCB; ++b) bv4[b] = (double)bm[s4][b];
 #pragma unroll
             for (int b = 0; b < NCB; ++b) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av4, bv4[b], acc[b], 0, 0, 0);
+            if constexpr (DM == 1 || DM == 2) {
+                const double x = k_ok ? bv4[0] : 0.0;  // the zeroed depths must not enter this product either
+                dacc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, bv4[DM == 2 ? NCB - 1 : 0], dacc[0], 0, 0, 0);
+            } else if constexpr (DM == 3) {
 #pragma unroll
-            for (int q = 0; q < DPW; ++q) {
-                if (wave + 4 * q < NDB) {
-                    // the zeroed depth of av4 is not in this product: mask x
-                    double x = bv4[0], y = bv4[0];
+                for (int q = 0; q < DPW; ++q) {
+                    if (wr + 4 * q < NDB) {
+                        double x = bv4[0], y = bv4[0];
 #pragma unroll
-                    for (int b = 1; b < NCB; ++b) {
-                        x = (dbi[q] == b) ? bv4[b] : x;
-                        y = (dbj[q] == b) ? bv4[b] : y;
+                        for (int b = 1; b < NCB; ++b) {
+                            x = (dbi[q] == b) ? bv4[b] : x;
+                            y = (dbj[q] == b) ? bv4[b] : y;
+                        }
+                        x = k_ok ? x : 0.0;
+                        dacc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, dacc[q], 0, 0, 0);
                     }
-                    x = ch * kKc + 4 * s4 + lk < K ? x : 0.0;
-                    dacc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, dacc[q], 0, 0, 0);
                 }
             }
         }
     };
-    // chunk c sits in register slot c % DEP; the slot refilled at each step
-    // is the one computed at the step before
-    for (int ch0 = 0; ch0 < nch; ch0 += DEP) {
+    // the t-th chunk of this K group (ch = kg + KG t) sits in register slot
+    // t % DEP; the slot refilled at each step is the one computed at the
+    // step before
+    auto kloop = [&](auto dm_c) {
+        for (int t0 = 0; kg + KG * t0 < nch; t0 += DEP) {
 #pragma unroll
-        for (int s = 0; s < DEP; ++s) {
-            const int ch = ch0 + s;
-            if (ch >= nch) break;
-            load_chunk(ch + DEP - 1, av[(s + DEP - 1) % DEP], bv[(s + DEP - 1) % DEP]);
-            compute(ch, av[s], bv[s]);
+            for (int s = 0; s < DEP; ++s) {
+                const int ch = kg + KG * (t0 + s);
+                if (ch >= nch) break;
+                load_chunk(ch + KG * (DEP - 1), av[(s + DEP - 1) % DEP], bv[(s + DEP - 1) % DEP]);
+                if (ch * kKc + kKc <= K) compute(std::false_type{}, dm_c, ch, av[s], bv[s]);
+                else compute(std::true_type{}, dm_c, ch, av[s], bv[s]);
+            }
         }
-    }
+    };
+    if (NCB > 2) kloop(std::integral_constant<int, 3>{});
+    else if (dm == 1) kloop(std::integral_constant<int, 1>{});
+    else if (dm == 2) kloop(std::integral_constant<int, 2>{});
+    else kloop(std::integral_constant<int, 0>{});
 #pragma unroll
     for (int q = 0; q < CR; ++q) {
         const int e = tid + q * kThreads, i = e % kRW, c = e / kRW;
@@ -193,27 +234,31 @@ __global__ __launch_bounds__(kThreads) void potrf_panel_kernel(T* P, int64_t rs,
 #pragma unroll
     for (int q = 0; q < CD; ++q) {
         const int e = tid + q * kThreads, i = e % NB, c = e / NB;
-        L[i * LD + c] = (i < kb && c <= i) ? (double)cur_d[q] : (i == c ? 1.0 : 0.0);
+        if (e < NB * NB) L[i * LD + c] = (i < kb && c <= i) ? (double)cur_d[q] : (i == c ? 1.0 : 0.0);
     }
     __syncthreads();  // the block column is in X and L
-    // subtract: lane l, item r of a 16x16 tile is element (4r + l/16, l%16)
-    if (K > 0) {
+    // subtract, one K group after the other: lane l, item r of a 16x16 tile
+    // is element (4r + l/16, l%16)
 #pragma unroll
-        for (int b = 0; b < NCB; ++b)
+    for (int g = 0; g < KG; ++g) {
+        if (K > 0 && kg == g) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) X[(16 * wave + 4 * r + lk) * LD + 16 * b + li] -= acc[b][r];
+            for (int b = 0; b < NCB; ++b)
 #pragma unroll
-        for (int q = 0; q < DPW; ++q) {
-            if (wave + 4 * q < NDB) {
+                for (int r = 0; r < 4; ++r) X[(16 * wr + 4 * r + lk) * LD + 16 * pblk[b] + li] -= acc[b][r];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int i = 16 * dbi[q] + 4 * r + lk, c = 16 * dbj[q] + li;
-                    if (c <= i && i < kb) L[i * LD + c] -= dacc[q][r];  // the identity padding stays
+            for (int q = 0; q < DPW; ++q) {
+                if (wr + 4 * q < NDB) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int i = 16 * dbi[q] + 4 * r + lk, c = 16 * dbj[q] + li;
+                        if (c <= i && i < kb) L[i * LD + c] -= dacc[q][r];  // the identity padding stays
+                    }
                 }
             }
         }
+        __syncthreads();
     }
-    __syncthreads();
 
     if constexpr (NB <= 32) {
         // 2+3. wavefronts 0.. hold the diagonal block in lanes 0..NB-1 and
@@ -272,6 +317,7 @@ __global__ __launch_bounds__(kThreads) void potrf_panel_kernel(T* P, int64_t rs,
             }
         }
 
+        if (wave >= 4) return;
         // 3. own rows: X L^T = A21.  Wave w solves rows [16w, 16w+16), lanes
         // 4p..4p+3 row p; lane q of a quad holds columns 4s + q.  The padded
         // columns come out zero and are not stored.
@@ -333,14 +379,14 @@ rmh_status launch_potrf_panel(T* P, int64_t rs, int64_t cs, int64_t rows, int64_
     const unsigned grid = (unsigned)std::max<int64_t>(1, (below + kRW - 1) / kRW);
     const int k = (int)K;
     if (kb <= 16)
-        hipLaunchKernelGGL((potrf_panel_kernel<T, 16>), dim3(grid), dim3(kThreads), 0, stream, P, rs, cs, rows, k, kb,
-                           info, info_base, dfac, ldf);
+        hipLaunchKernelGGL((potrf_panel_kernel<T, 16>), dim3(grid), dim3(panel_threads(16)), 0, stream, P, rs, cs, rows,
+                           k, kb, info, info_base, dfac, ldf);
     else if (kb <= 32)
-        hipLaunchKernelGGL((potrf_panel_kernel<T, 32>), dim3(grid), dim3(kThreads), 0, stream, P, rs, cs, rows, k, kb,
-                           info, info_base, dfac, ldf);
+        hipLaunchKernelGGL((potrf_panel_kernel<T, 32>), dim3(grid), dim3(panel_threads(32)), 0, stream, P, rs, cs, rows,
+                           k, kb, info, info_base, dfac, ldf);
     else if (kb <= 64)
-        hipLaunchKernelGGL((potrf_panel_kernel<T, 64>), dim3(grid), dim3(kThreads), 0, stream, P, rs, cs, rows, k, kb,
-                           info, info_base, dfac, ldf);
+        hipLaunchKernelGGL((potrf_panel_kernel<T, 64>), dim3(grid), dim3(panel_threads(64)), 0, stream, P, rs, cs, rows,
+                           k, kb, info, info_base, dfac, ldf);
     else
         return fail(RMH_STATUS_INVALID_VALUE, "potrf_panel: block wider than 64");
     return check_launch("potrf_panel_kernel");
