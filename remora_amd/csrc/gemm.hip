@@ -82,7 +82,32 @@ struct KParams {
     // triangular operands: a tile whose rows start at m0 only accumulates
     // k >= m0 (op(A)(i, k) = 0 for k < i)
     int k_from_row;
+    // every element offset inside a staged tile fits 32 bits (in bytes):
+    // full stages load from a wave-uniform base plus fixed per-lane offsets
+    int fast32;
 };
+
+// word 3 of a raw (untyped) buffer descriptor on gfx9-family parts
+constexpr int kBufWord3 = 0x00020000;
+// a wave-uniform pointer, moved to scalar registers (a buffer resource must
+// be scalar; the compiler cannot always prove the uniformity itself)
+template <typename T>
+__device__ __forceinline__ const T* uniform_ptr(const T* p) {
+    const uint64_t u = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+    return reinterpret_cast<const T*>(((uint64_t)hi << 32) | lo);
+}
+template <typename T>
+__device__ __forceinline__ T buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <>
+__device__ __forceinline__ double buffer_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+template <>
+__device__ __forceinline__ float buffer_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
 
 constexpr int kThreads = 256;
 constexpr int kPad = 2;  // LDS row padding (elements): 16 gave heavy bank conflicts on the operand reads
@@ -166,7 +191,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     } else {
         lower_trapezoid_tile(pid, p.tiles_m, tn, tm);
     }
-    const int64_t m0 = tm * BM, n0 = tn * BN;
+    // the tile origin is uniform: keep it in scalar registers so that the
+    // operand bases of the K loop are scalar too
+    const int64_t m0 = __builtin_amdgcn_readfirstlane((int)(tm * BM));
+    const int64_t n0 = __builtin_amdgcn_readfirstlane((int)(tn * BN));
     if (p.k_from_row) {
         const int64_t kb = min(p.K, m0);  // m0 is a multiple of BK
         p.A += TA ? kb : kb * p.lda;
@@ -205,12 +233,45 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     int st_kmax = 0;    // last valid k inside the staged tile
     const int mmax = (int)min((int64_t)BM, M - m0) - 1;  // last valid row / column of this tile
     const int nmax = (int)min((int64_t)BN, N - n0) - 1;
+    // The MFMA and VALU pipes of a SIMD do not overlap, so a stage's VALU
+    // work adds to its MFMA time.  Full stages (all BK depths inside K, and
+    // 32-bit offsets) therefore load from a wave-uniform base plus per-lane
+    // byte offsets computed once per tile (rows and columns clamped), and
+    // interior tiles store them to LDS unmasked.
+    uint32_t aoff[A_PER], boff[B_PER];
+    if (SYM == 0) {
+#pragma unroll
+        for (int r = 0; r < A_PER; ++r) {
+            const int cm = min(a_m0 + r * AM_STEP, mmax), ck = a_k0 + r * AK_STEP;
+            aoff[r] = (uint32_t)(TA ? ck + cm * lda32 : cm + ck * lda32) * (uint32_t)sizeof(T);
+        }
+#pragma unroll
+        for (int r = 0; r < B_PER; ++r) {
+            const int cn = min(b_n0 + r * BN_STEP, nmax), ck = b_k0 + r * BK_STEP;
+            boff[r] = (uint32_t)(TB ? cn + ck * ldb32 : ck + cn * ldb32) * (uint32_t)sizeof(T);
+        }
+    }
+    const bool interior = mmax == BM - 1 && nmax == BN - 1;
     // Loads go to clamped (always valid) addresses and are not inspected
     // here: zeroing the out-of-range elements happens at the LDS store, after
     // the MFMAs of the current stage, so the loads stay in flight meanwhile.
     auto load_tiles = [&](int64_t k0) {
         const int kmax = (int)min((int64_t)BK, K - k0) - 1;  // >= 0
         st_kmax = kmax;
+        if (SYM == 0 && p.fast32 && kmax == BK - 1) {
+            // buffer loads: the stage's tile origin in the (scalar) resource,
+            // the fixed lane offsets in the VGPR operand
+            const T* ab = uniform_ptr(TA ? p.A + k0 + m0 * p.lda : p.A + m0 + k0 * p.lda);
+            const T* bb = uniform_ptr(TB ? p.B + n0 + k0 * p.ldb : p.B + k0 + n0 * p.ldb);
+            const __amdgpu_buffer_rsrc_t ar = __builtin_amdgcn_make_buffer_rsrc((void*)ab, (short)0, 0x7fffffff, kBufWord3);
+            const __amdgpu_buffer_rsrc_t br = __builtin_amdgcn_make_buffer_rsrc((void*)bb, (short)0, 0x7fffffff, kBufWord3);
+            a_mode = 0;
+#pragma unroll
+            for (int r = 0; r < A_PER; ++r) ra[r] = buffer_load<T>(ar, aoff[r]);
+#pragma unroll
+            for (int r = 0; r < B_PER; ++r) rb[r] = buffer_load<T>(br, boff[r]);
+            return;
+        }
         const int64_t ka = k0 + kb0;  // absolute depth of A (differs from k0 only for SYM splits)
         const T* abase = TA ? p.A + ka + m0 * p.lda : p.A + m0 + ka * p.lda;
         const T* bbase = TB ? p.B + n0 + k0 * p.ldb : p.B + k0 + n0 * p.ldb;
@@ -256,6 +317,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     };
     auto store_tiles = [&](int buf) {
         const int kmax = st_kmax;
+        if (SYM == 0 && interior && kmax == BK - 1) {
+#pragma unroll
+            for (int r = 0; r < A_PER; ++r) As[buf][(a_k0 + r * AK_STEP) * LDA_S + a_m0 + r * AM_STEP] = ra[r];
+#pragma unroll
+            for (int r = 0; r < B_PER; ++r) Bs[buf][(b_k0 + r * BK_STEP) * LDB_S + b_n0 + r * BN_STEP] = rb[r];
+            return;
+        }
         if (SYM != 0 && a_mode == 1) {
 #pragma unroll
             for (int r = 0; r < A_PER; ++r) {
@@ -578,6 +646,12 @@ rmh_status launch_cfg(const GemmDesc<T>& d, hipStream_t stream) {
     p.k_chunk = d.k_chunk;
     p.split_ws = d.split_ws;
     p.k_from_row = d.k_from_row ? 1 : 0;
+    {
+        const int64_t a_max = d.transA ? (BK - 1) + (int64_t)(BM - 1) * d.lda : (BM - 1) + (int64_t)(BK - 1) * d.lda;
+        const int64_t b_max = d.transB ? (BN - 1) + (int64_t)(BK - 1) * d.ldb : (BK - 1) + (int64_t)(BN - 1) * d.ldb;
+        const int64_t lim = ((int64_t)1 << 31) / (int64_t)sizeof(T) - 1;  // buffer range: 2^31 - 1 bytes
+        p.fast32 = (a_max < lim && b_max < lim) ? 1 : 0;
+    }
     int64_t ntiles;
     if (PART == 0) {
         ntiles = p.tiles_m * p.tiles_n;
@@ -638,6 +712,7 @@ rmh_status launch_symm_cfg(const GemmDesc<T>& d, hipStream_t stream) {
     p.batch_ny = 1;
     p.sAy = p.sAz = p.sBy = p.sBz = p.sCy = p.sCz = 0;
     p.k_from_row = 0;
+    p.fast32 = 0;
     int splits = 1;
     p.k_chunk = d.K;
     p.split_ws = nullptr;
