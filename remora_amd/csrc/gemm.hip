@@ -477,7 +477,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     }
     int64_t tm, tn;
     lower_trapezoid_tile(pid, p.tiles_n, tm, tn);
-    const int64_t m0 = tm * BM, n0 = tn * BN;
+    const int64_t m0 = __builtin_amdgcn_readfirstlane((int)(tm * BM));
+    const int64_t n0 = __builtin_amdgcn_readfirstlane((int)(tn * BN));
     const int64_t M = p.M, N = p.N, K = p.K;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
@@ -489,9 +490,25 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     const int lda32 = (int)p.lda, ldb32 = (int)p.ldb;
     float ra[16], rb[16];
     int st_kmax = 0;
+    // full stages load through buffer descriptors: one fixed lane offset,
+    // the depth step in the scalar offset (no per-element address VALU work)
+    const bool buf_ok = (int64_t)(BM + BK * p.lda) * 4 < ((int64_t)1 << 30) && (int64_t)(BN + BK * p.ldb) * 4 < ((int64_t)1 << 30);
+    const uint32_t a_vo = (uint32_t)(min(srow, mmax) + sk0 * lda32) * 4u, b_vo = (uint32_t)(min(srow, nmax) + sk0 * ldb32) * 4u;
     auto load_tiles = [&](int64_t k0) {
         const int kmax = (int)min((int64_t)BK, K - k0) - 1;
         st_kmax = kmax;
+        if (buf_ok && kmax == BK - 1) {
+            const __amdgpu_buffer_rsrc_t ar =
+                __builtin_amdgcn_make_buffer_rsrc((void*)uniform_ptr(p.A + m0 + k0 * p.lda), (short)0, 0x7fffffff, kBufWord3);
+            const __amdgpu_buffer_rsrc_t br =
+                __builtin_amdgcn_make_buffer_rsrc((void*)uniform_ptr(p.B + n0 + k0 * p.ldb), (short)0, 0x7fffffff, kBufWord3);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                ra[r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(ar, (int)a_vo, r * lda32 * 4, 0));
+                rb[r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(br, (int)b_vo, r * ldb32 * 4, 0));
+            }
+            return;
+        }
         const float* abase = p.A + m0 + k0 * p.lda;
         const float* bbase = p.B + n0 + k0 * p.ldb;
         const int cm = min(srow, mmax), cn = min(srow, nmax);
