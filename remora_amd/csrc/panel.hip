@@ -86,7 +86,7 @@ __global__ __launch_bounds__(panel_threads(NB)) void potrf_panel_kernel(T* P, in
     __shared__ double L[NB * LD];  // updated diagonal block, then its factor
     __shared__ double X[kRW * LD]; // updated own rows
     __shared__ double rdiag[NB];
-    __shared__ __align__(16) double col[4 * NB];  // pivot-column buffer per wavefront
+    __shared__ __align__(16) double col[4 * NB];  // pivot-column buffers (two per factoring wavefront)
     __shared__ int failed;
 
     const int tid = threadIdx.x, lane = tid & 63;
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(panel_threads(NB)) void potrf_panel_kernel(T* P, in
         double r[NB];
 #pragma unroll
         for (int c = 0; c < NB; ++c) r[c] = src[c];
-        const int bad = wave_chol_rows<NB>(r, &col[wave * NB], lane);
+        const int bad = wave_chol_rows<NB>(r, &col[wave * 2 * NB], lane);
         if (bad >= 0) {
             // keep the first failure: later launches run on garbage but must not overwrite it
             if (blockIdx.x == 0 && tid == 0)

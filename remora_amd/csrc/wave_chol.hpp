@@ -110,15 +110,27 @@ __device__ __forceinline__ int wave_potrf_lds(double* L, double* rdiag, double* 
 // with the identity, as for wave_potrf), every other lane holds one row a of
 // the block column below it.  The elimination applied to the block rows is
 // applied to those rows too, so on return lanes i < NB hold row i of the
-// factor L and the other lanes hold x with x L^T = a.  Pivot columns move
-// through col[] (NB doubles of LDS) except the next one, which moves by
-// readlane.  Returns the first non-positive (or NaN) pivot, -1 if none.
+// factor L and the other lanes hold x with x L^T = a.  Returns the first
+// non-positive (or NaN) pivot, -1 if none.
+//
+// Step j: the pivot (from lane j) and its 1/sqrt, the scaled column j, the
+// next column j+1 updated at once (its entries move by readlane), and the
+// other columns updated one step late: step j-1's pivot column is read back
+// from LDS (col: 2 x NB doubles, one buffer per parity of j) at the start of
+// step j, so those loads and FMAs fill the latency of step j's pivot chain.
 template <int NB>
 __device__ __forceinline__ int wave_chol_rows(double (&r)[NB], double* col, int lane) {
     static_assert(NB <= 32, "block rows and solved rows share the wavefront");
-    int bad = -1;  // wave-uniform
+    int bad = -1;    // wave-uniform
+    double pj = 0.0;  // this lane's entry of the previous pivot column
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
+        // step j-1's updates of columns j+1.. (column j took it at step j-1)
+        if (j >= 1) {
+            const double* cprev = col + ((j - 1) & 1) * NB;
+#pragma unroll
+            for (int c = j + 1; c < NB; ++c) r[c] = fma(-pj, cprev[c], r[c]);
+        }
         const double d = lane_bcast(r[j], j);
         bad = (bad < 0 && !(d > 0.0)) ? j : bad;
         double y = __builtin_amdgcn_rsq(d);
@@ -128,14 +140,12 @@ __device__ __forceinline__ int wave_chol_rows(double (&r)[NB], double* col, int 
         double sj = d * y;
         sj = fma(fma(-sj, sj, d), 0.5 * y, sj);
         r[j] = (lane == j) ? sj : r[j] * y;
+        pj = r[j];
         if (j + 1 < NB) {
-            if (j + 2 < NB && lane < NB) col[lane] = r[j];
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (j + 2 < NB && lane < NB) col[(j & 1) * NB + lane] = r[j];
             r[j + 1] = fma(-r[j], lane_bcast(r[j], j + 1), r[j + 1]);
-#pragma unroll
-            for (int c = j + 2; c < NB; ++c) r[c] = fma(-r[j], col[c], r[c]);
+            // the next step reads what the other lanes wrote (one
+            // wavefront, LDS operations in program order)
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
