@@ -4,7 +4,7 @@
 // A recursive solve splits into ever smaller GEMMs whose output is only
 // nrhs columns wide: they cannot fill 256 CUs, and hundreds of dependent
 // launches dominate.  Here the triangular matrix is cut into nbd-row blocks
-// (nbd = 512 for large n) and the inverses of its diagonal blocks are formed
+// (at n = 16384: 1024 rows in FP64, 2048 in FP32) and the inverses of its diagonal blocks are formed
 // first, hierarchically and in parallel: the 64-block inverses in one
 // launch, then for s = 64, 128, ... the 2s-block inverse from its halves,
 //     inv([P 0; Q R]) = [inv(P) 0; -inv(R) Q inv(P)  inv(R)],
@@ -34,8 +34,15 @@ struct SView {
     }
 };
 
+// Block size of the substitution.  Each block costs four dependent launches
+// (two split-K GEMMs and their reductions), so fewer, larger blocks shorten
+// the latency-bound sweep, while the inverses grow as nbd^2 per row.  FP32
+// (the refinement solves of dsposv, three or four per call) takes 2048-row
+// blocks above n = 8192: 1.0-1.4 ms less per mixed solve at n = 16384; FP64
+// potrs, built and swept once, gains nothing from them.
+template <typename T>
 int64_t choose_nbd(int64_t na) {
-    if (na > 8192) return 1024;
+    if (na > 8192) return sizeof(T) == 4 ? 2048 : 1024;
     if (na > 4096) return 512;
     if (na > 1024) return 256;
     return 128;
@@ -125,7 +132,7 @@ bool trsm_blocked_applies(int64_t na, int64_t other) { return na >= 512 && other
 
 template <typename T>
 size_t trsm_inverse_workspace(int64_t na, int64_t other) {
-    const int64_t nbd = choose_nbd(na);
+    const int64_t nbd = choose_nbd<T>(na);
     const size_t d_el = (size_t)na * nbd, w_el = (size_t)na * nbd / 2 + 64 * 64;
     const size_t p_el = partial_elems<T>(na, other, nbd);
     return ((d_el + 63) / 64 * 64 + (w_el + 63) / 64 * 64 + p_el) * sizeof(T) + 3 * 256;
@@ -134,7 +141,7 @@ size_t trsm_inverse_workspace(int64_t na, int64_t other) {
 template <typename T>
 rmh_status trsm_prepare(const T* A, int64_t lda, int64_t na, bool upper, bool unit, void* ws, TrsmInverses<T>* inv,
                         hipStream_t s) {
-    const int64_t nbd = choose_nbd(na);
+    const int64_t nbd = choose_nbd<T>(na);
     const size_t d_el = (size_t)na * nbd, w_el = (size_t)na * nbd / 2 + 64 * 64;
     T* D = static_cast<T*>(ws);
     T* W = D + (d_el + 63) / 64 * 64;

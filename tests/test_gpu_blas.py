@@ -246,3 +246,24 @@ def test_invalid_arguments(handle):
     assert b"lda" in lib.rmh_last_error()
     st = lib.rmh_dpotrf(None, 0, 4, None, 4, None, None)
     assert st == capi.INVALID_HANDLE
+
+
+@pytest.mark.parametrize("left,upper,trans", [(True, False, False), (True, False, True), (True, True, False),
+                                              (False, False, True)])
+def test_strsm_large_blocks(handle, left, upper, trans):
+    """FP32 blocked solve above n = 8192 (2048-row blocks; 9000 leaves a
+    808-row last block), 64 right-hand sides: componentwise residual."""
+    import torch
+    from remora_amd import linalg as la
+    n, other = 9000, 64
+    L = synth.uniform(5, 0, n * n, -1, 1).reshape(n, n) / n + np.eye(n) * 2.0
+    A = (np.tril(L) if not upper else np.triu(L)).astype(np.float32).astype(np.float64)
+    B = synth.uniform(5, 1, n * other, -1, 1).reshape((n, other) if left else (other, n))
+    B = B.astype(np.float32).astype(np.float64)
+    dB = dev(B, dtype=torch.float32)
+    la.trsm(handle, dev(A, dtype=torch.float32), dB, left=left, upper=upper, trans=trans)
+    X = host(dB)
+    opT = A.T if trans else A
+    resid = (opT @ X - B) if left else (X @ opT - B)
+    scale = (np.abs(opT) @ np.abs(X)) if left else (np.abs(X) @ np.abs(opT))
+    assert np.max(np.abs(resid) / scale) < 1e-4

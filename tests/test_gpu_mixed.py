@@ -118,3 +118,16 @@ def test_dsposv_bf16x3(handle, n):
         dX2 = dev(np.zeros((n, 16)))
         it32, _ = la.dsposv(handle, dev(A), dev(B), dX2, prec="fp32")
         assert it >= it32
+
+
+def test_dsposv_fp32_large_blocks(handle):
+    """n = 9000 (> 8192): the FP32 refinement solves run on 2048-row blocks
+    with a ragged last block; the result still reaches 1e-14."""
+    from remora_amd import linalg as la
+    n, nrhs = 9000, 64
+    A, B = _rbf_problem(n, nrhs)
+    dX = dev(np.zeros((n, nrhs)))
+    it, berr = la.dsposv(handle, dev(A), dev(B), dX, tol=1e-14, max_iter=30)
+    assert it >= 1, it
+    assert berr <= 1e-14
+    assert synth.backward_error(A, host(dX), B) <= 2e-14
