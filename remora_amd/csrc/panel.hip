@@ -24,12 +24,12 @@
 //    rows in the other lanes, and runs the right-looking elimination of the
 //    block on all of them (wave_chol_rows): the block lanes end with the
 //    factor L, the row lanes with x, x L^T = a.  Every wavefront and
-//    workgroup forms the same factor, so none waits for another.  (Wider
-//    blocks factor in wavefront 0 and solve the rows afterwards, four lanes
-//    per row.)  Workgroup 0 writes the factor to a side buffer rather than
-//    over the block, which the other workgroups may still be reading; the
-//    driver copies all diagonal factors into place once the factorisation
-//    is done.
+//    workgroup forms the same factor, so none waits for another.  (64-wide
+//    blocks run this twice, on their two 32-wide halves, with the rank-32
+//    update of the second half between.)  Workgroup 0 writes the factor to
+//    a side buffer rather than over the block, which the other workgroups
+//    may still be reading; the driver copies all diagonal factors into
+//    place once the factorisation is done.
 // The arithmetic is FP64 whatever the storage type (the mixed-precision
 // factorisation keeps its diagonal blocks in FP64).
 #include <type_traits>
@@ -57,17 +57,6 @@ __device__ __forceinline__ void lower_block(int d, int& bi, int& bj) {
     bj = d;
 }
 
-// v of lane (lane ^ 1) and of lane (lane ^ 2) within each quad of lanes
-template <int CTRL>
-__device__ __forceinline__ double quad_perm(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, false);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-constexpr int kQuadXor1 = 0xB1;  // quad_perm [1,0,3,2]
-constexpr int kQuadXor2 = 0x4E;  // quad_perm [2,3,0,1]
-
 template <typename T, int NB>
 __global__ __launch_bounds__(panel_threads(NB)) void potrf_panel_kernel(T* P, int64_t rs, int64_t cs, int64_t rows, int K,
                                                                int kb, int64_t* info, int64_t info_base, T* dfac,
@@ -80,12 +69,10 @@ __global__ __launch_bounds__(panel_threads(NB)) void potrf_panel_kernel(T* P, in
     constexpr int LD = NB + 1;
     constexpr int CD = (NB * NB + kThreads - 1) / kThreads, CR = kRW * NB / kThreads;  // block-column elements per thread
     constexpr int DEP = 2;                    // K chunks in flight per wavefront (deeper only grows the code)
-    constexpr int NQ = NB / 4;                // solve: columns per lane
     static_assert(NB % 16 == 0 && CR * kThreads == kRW * NB, "unsupported block width");
 
     __shared__ double L[NB * LD];  // updated diagonal block, then its factor
     __shared__ double X[kRW * LD]; // updated own rows
-    __shared__ double rdiag[NB];
     __shared__ __align__(16) double col[4 * NB];  // pivot-column buffers (two per factoring wavefront)
     __shared__ int failed;
 
@@ -295,57 +282,100 @@ __global__ __launch_bounds__(panel_threads(NB)) void potrf_panel_kernel(T* P, in
                 if (c <= lane) dfac[lane + c * ldf] = (T)r[c];
         }
     } else {
-        // 2. factor the (identity-padded) diagonal block in wavefront 0
-        if (tid < NB) {
-            const int bad = wave_potrf_lds<NB, LD>(L, rdiag, col, tid);
-            if (tid == 0) failed = bad;
+        // 64-wide block column as two 32-wide halves, each factored with the
+        // rows along (wave_chol_rows): half 1 solves the lower half of the
+        // diagonal block and the own rows, then both take the rank-32
+        // update of half 1, then half 2 solves the own rows.  The whole
+        // 64 x 64 lower block goes to the side buffer.
+        constexpr int H = NB / 2;
+        double* cbuf = &col[wave * 2 * H];
+        // half 1: wave 0 holds rows 0..63 of the block (pivot rows 0..31 and
+        // the rows 32..63 below them), waves 1 and 2 the pivot rows and 32
+        // own rows each
+        {
+            const int orow = (wave - 1) * H + lane - H;
+            const bool own = wave >= 1 && wave < 3 && lane >= H;
+            const double* src = own ? &X[orow * LD] : &L[lane * LD];
+            double r[H];
+#pragma unroll
+            for (int c = 0; c < H; ++c) r[c] = src[c];
+            __syncthreads();  // every wavefront holds its rows before wave 0 writes the block back
+            if (wave < 3) {
+                const int bad = wave_chol_rows<H>(r, cbuf, lane);
+                if (wave == 0 && lane == 0) failed = bad;
+                // wave 0 writes the first half of the block (its pivot rows
+                // and the rows below them), waves 1 and 2 their own rows
+                double* dst = (wave == 0) ? &L[lane * LD] : (own ? &X[orow * LD] : nullptr);
+                if (dst != nullptr) {
+#pragma unroll
+                    for (int c = 0; c < H; ++c) dst[c] = r[c];
+                }
+            }
         }
         __syncthreads();
-        const int bad = failed;
-        if (bad >= 0) {
-            // keep the first failure: later launches run on garbage but must not overwrite it
+        if (failed >= 0) {
             if (blockIdx.x == 0 && tid == 0)
-                atomicCAS(reinterpret_cast<unsigned long long*>(info), 0ull, (unsigned long long)(info_base + bad + 1));
+                atomicCAS(reinterpret_cast<unsigned long long*>(info), 0ull, (unsigned long long)(info_base + failed + 1));
             return;
         }
-        // the factor of the diagonal block goes to the side buffer: every
-        // workgroup of this launch still reads the block's original values
-        if (blockIdx.x == 0) {
-            for (int e = tid; e < kb * kb; e += kThreads) {
-                const int i = e % kb, c = e / kb;
-                if (c <= i) dfac[i + c * ldf] = (T)L[i * LD + c];
+        // rank-32 update of the second half: rows 32..63 of the block (lower
+        // part) and the own rows, a[:, 32 + c] -= x[:, 0:32] . L[32 + c, 0:32]
+        for (int e = tid; e < (kRW + H) * H; e += kThreads) {
+            const int i = e % (kRW + H), c = e / (kRW + H);
+            double* row = i < H ? &L[(H + i) * LD] : &X[(i - H) * LD];
+            if (i < H && c > i) continue;  // upper part of the second diagonal block
+            const double* lr = &L[(H + c) * LD];
+            double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+            for (int t = 0; t < H; t += 2) {
+                s0 = fma(row[t], lr[t], s0);
+                s1 = fma(row[t + 1], lr[t + 1], s1);
             }
+            row[H + c] -= s0 + s1;
         }
-
-        if (wave >= 4) return;
-        // 3. own rows: X L^T = A21.  Wave w solves rows [16w, 16w+16), lanes
-        // 4p..4p+3 row p; lane q of a quad holds columns 4s + q.  The padded
-        // columns come out zero and are not stored.
-        const int srow = 16 * wave + (lane >> 2), q = lane & 3;
-        double xs[NQ];
+        __syncthreads();
+        // half 2: waves 0 and 1 hold the second diagonal block (lanes 0..31)
+        // and 32 own rows each
+        if (wave < 2) {
+            const int orow = wave * H + lane - H;
+            const double* src = lane < H ? &L[(H + lane) * LD + H] : &X[orow * LD + H];
+            double r[H];
 #pragma unroll
-        for (int s = 0; s < NQ; ++s) xs[s] = X[srow * LD + 4 * s + q];
-#pragma unroll
-        for (int c = 0; c < NB; ++c) {
-            const int sc = c / 4, oc = c % 4;
-            const double* lrow = &L[c * LD + q];
-            double p0 = 0.0, p1 = 0.0;  // two chains
-#pragma unroll
-            for (int s = 0; s < sc; ++s) {
-                if (s & 1) p1 = fma(xs[s], lrow[4 * s], p1);
-                else p0 = fma(xs[s], lrow[4 * s], p0);
+            for (int c = 0; c < H; ++c) r[c] = src[c];
+            const int bad = wave_chol_rows<H>(r, cbuf, lane);
+            if (bad >= 0) {
+                if (blockIdx.x == 0 && tid == 0)
+                    atomicCAS(reinterpret_cast<unsigned long long*>(info), 0ull,
+                              (unsigned long long)(info_base + H + bad + 1));
+                return;
             }
-            if (oc > 0) p1 = (q < oc) ? fma(xs[sc], lrow[4 * sc], p1) : p1;  // columns 4sc + q < c
-            double part = p0 + p1;
-            part += quad_perm<kQuadXor1>(part);
-            part += quad_perm<kQuadXor2>(part);
-            xs[sc] = (q == oc) ? (xs[sc] - part) * rdiag[c] : xs[sc];
-        }
-        if (srow < nown) {
-            const int64_t i = r0 + srow;
+            if (lane >= H && orow < nown) {
+                const int64_t i = r0 + orow;
 #pragma unroll
-            for (int s = 0; s < NQ; ++s)
-                if (4 * s + q < kb) P[i * rs + (int64_t)(K + 4 * s + q) * cs] = (T)xs[s];
+                for (int c = 0; c < NB; ++c) {
+                    const double v = c < H ? X[orow * LD + c] : r[c - H];
+                    if (c < kb) P[i * rs + (int64_t)(K + c) * cs] = (T)v;
+                }
+            }
+            // the factor of the diagonal block goes to the side buffer: every
+            // workgroup of this launch still reads the block's original values
+            if (blockIdx.x == 0 && wave == 0) {
+                if (lane < H) {
+                    const int i = H + lane;
+                    if (i < kb) {
+#pragma unroll
+                        for (int c = 0; c < NB; ++c) {
+                            const double v = c < H ? L[i * LD + c] : r[c - H];
+                            if (c <= i) dfac[i + c * ldf] = (T)v;
+                        }
+                    }
+                }
+            }
+            if (blockIdx.x == 0 && wave == 1 && lane < H && lane < kb) {
+#pragma unroll
+                for (int c = 0; c < H; ++c)
+                    if (c <= lane) dfac[lane + c * ldf] = (T)L[lane * LD + c];
+            }
         }
     }
 }

@@ -59,52 +59,6 @@ __device__ __forceinline__ int wave_potrf(double* L, double* rdiag, int lane) {
     return bad;
 }
 
-// The same factorisation with the pivot column passed through LDS: at step
-// j every lane writes its entry of column j to col[] (NB doubles of LDS,
-// 16-byte aligned) and reads the entries it needs back as broadcast reads,
-// instead of one readlane pair per entry.  Only the next column (j + 1),
-// which the next pivot depends on, still moves by readlane, so the pivot
-// chain does not wait for the LDS round trip.  Same contract as wave_potrf.
-template <int NB, int LD>
-__device__ __forceinline__ int wave_potrf_lds(double* L, double* rdiag, double* col, int lane) {
-    int bad = -1;  // wave-uniform
-    double r[NB], piv = 1.0;
-#pragma unroll
-    for (int c = 0; c < NB; ++c) r[c] = L[lane * LD + c];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        const double d = lane_bcast(r[j], j);
-        bad = (bad < 0 && !(d > 0.0)) ? j : bad;
-        double y = __builtin_amdgcn_rsq(d);
-        const double hd = 0.5 * d;
-        y = fma(y, fma(-hd * y, y, 0.5), y);
-        y = fma(y, fma(-hd * y, y, 0.5), y);
-        double sj = d * y;
-        sj = fma(fma(-sj, sj, d), 0.5 * y, sj);
-        r[j] = (lane == j) ? sj : r[j] * y;
-        piv = (lane == j) ? sj : piv;
-        if (j + 1 < NB) {
-            if (j + 2 < NB) col[lane] = r[j];
-            // the reads below see the other lanes' writes: one wavefront,
-            // LDS operations in program order
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            r[j + 1] = fma(-r[j], lane_bcast(r[j], j + 1), r[j + 1]);
-#pragma unroll
-            for (int c = j + 2; c < NB; ++c) r[c] = fma(-r[j], col[c], r[c]);
-            // all reads of col[] done before the next step overwrites it
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < NB; ++c) L[lane * LD + c] = r[c];
-    rdiag[lane] = 1.0 / piv;
-    return bad;
-}
-
 // Right-looking Cholesky of an NB x NB diagonal block (NB <= 32) that also
 // solves the rows below it: lanes 0..NB-1 hold the rows of the block (padded
 // with the identity, as for wave_potrf), every other lane holds one row a of
