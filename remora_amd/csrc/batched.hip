@@ -90,17 +90,16 @@ struct Smem {
     int64_t item;
 };
 
-// The workgroup's LDS.  The per-system routines below are not inlined into
-// the persistent loop (each gets its own register allocation; the loop keeps
-// almost nothing live), and they reach the LDS through this variable so the
-// accesses stay LDS instructions.
+// The workgroup's LDS.  The per-system routines reach it through this
+// variable, so the accesses stay LDS instructions also where they are not
+// inlined into the persistent loop.
 template <typename T, int NB>
 __shared__ Smem<T, NB> g_bsm;
 
 // Factor the n x n matrix A (column-major, ld) in place.  Returns 0 or the
 // LAPACK info (k+1 of the first failing pivot).  Uniform across the block.
 template <typename T, int NB>
-__device__ __attribute__((noinline)) int64_t factor_one(T* A, int n, int64_t lda) {
+__device__ __forceinline__ int64_t factor_one_impl(T* A, int n, int64_t lda) {
     Smem<T, NB>& s = g_bsm<T, NB>;
     const int tid = threadIdx.x;
     typedef MfmaT<T> M;
@@ -221,7 +220,7 @@ __device__ __forceinline__ void diag_solve(const double* L, double* X, int nrhs,
 
 // Solve L L^T X = B in place (B column-major n x nrhs, ld ldb).
 template <typename T, int NB>
-__device__ __attribute__((noinline)) void solve_one(const T* A, int n, int64_t lda, T* B, int nrhs, int64_t ldb) {
+__device__ __forceinline__ void solve_one_impl(const T* A, int n, int64_t lda, T* B, int nrhs, int64_t ldb) {
     Smem<T, NB>& s = g_bsm<T, NB>;
     const int tid = threadIdx.x;
     constexpr int LD = Smem<T, NB>::LD, XL = kMaxRhs + 1;
@@ -290,8 +289,23 @@ __device__ __attribute__((noinline)) void solve_one(const T* A, int n, int64_t l
     }
 }
 
+// Out-of-line copies: each routine then gets its own register allocation
+// and the persistent loop keeps almost nothing live across the calls.
 template <typename T, int NB>
-__global__ __launch_bounds__(kThreads) void batched_posv_kernel(BArgs<T> a) {
+__device__ __attribute__((noinline)) int64_t factor_one(T* A, int n, int64_t lda) {
+    return factor_one_impl<T, NB>(A, n, lda);
+}
+template <typename T, int NB>
+__device__ __attribute__((noinline)) void solve_one(const T* A, int n, int64_t lda, T* B, int nrhs, int64_t ldb) {
+    solve_one_impl<T, NB>(A, n, lda, B, nrhs, ldb);
+}
+
+// The persistent loop.  FP64 calls the out-of-line routines (about 390
+// registers, one workgroup per CU).  FP32 inlines them into a kernel held
+// to 256 registers, two workgroups per CU: a few spills, but 11.2 against
+// 13.3 ms on the 4096-system benchmark, where FP64 loses 7% the same way.
+template <typename T, int NB, bool INL>
+__device__ __forceinline__ void batched_posv_body(const BArgs<T>& a) {
     Smem<T, NB>& s = g_bsm<T, NB>;
     const int tid = threadIdx.x;
     for (;;) {
@@ -307,8 +321,11 @@ __global__ __launch_bounds__(kThreads) void batched_posv_kernel(BArgs<T> a) {
             info = -1;
         } else if (n64 > 0) {
             const int n = (int)n64;
-            if (a.factor) info = factor_one<T, NB>(a.A[mat], n, a.lda[mat]);
-            if (info == 0 && a.nrhs > 0) solve_one<T, NB>(a.A[mat], n, a.lda[mat], a.B[mat], a.nrhs, a.ldb[mat]);
+            if (a.factor) info = INL ? factor_one_impl<T, NB>(a.A[mat], n, a.lda[mat]) : factor_one<T, NB>(a.A[mat], n, a.lda[mat]);
+            if (info == 0 && a.nrhs > 0) {
+                if constexpr (INL) solve_one_impl<T, NB>(a.A[mat], n, a.lda[mat], a.B[mat], a.nrhs, a.ldb[mat]);
+                else solve_one<T, NB>(a.A[mat], n, a.lda[mat], a.B[mat], a.nrhs, a.ldb[mat]);
+            }
         }
         if (tid == 0 && a.info != nullptr) a.info[mat] = info;
     }
@@ -323,16 +340,31 @@ __global__ __launch_bounds__(kThreads) void batched_posv_kernel(BArgs<T> a) {
 }
 
 template <typename T, int NB>
+__global__ __launch_bounds__(kThreads) void batched_posv_kernel(BArgs<T> a) {
+    batched_posv_body<T, NB, false>(a);
+}
+template <typename T, int NB>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void batched_posv_kernel_inl(BArgs<T> a) {
+    batched_posv_body<T, NB, true>(a);
+}
+// the kernel of a precision and block size
+template <typename T, int NB>
+constexpr auto batched_kernel() {
+    if constexpr (sizeof(T) == 4) return batched_posv_kernel_inl<T, NB>;
+    else return batched_posv_kernel<T, NB>;
+}
+
+template <typename T, int NB>
 rmh_status launch_batched(const BArgs<T>& a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((batched_posv_kernel<T, NB>), dim3((unsigned)grid), dim3(kThreads), 0, s, a);
+    hipLaunchKernelGGL((batched_kernel<T, NB>()), dim3((unsigned)grid), dim3(kThreads), 0, s, a);
     return check_launch("batched_posv_kernel");
 }
 
 template <typename T>
 int max_wg_per_cu(int nb) {
     int blocks = 0;
-    hipError_t e = nb == 16 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, batched_posv_kernel<T, 16>, kThreads, 0)
-                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, batched_posv_kernel<T, 32>, kThreads, 0);
+    hipError_t e = nb == 16 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, batched_kernel<T, 16>(), kThreads, 0)
+                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, batched_kernel<T, 32>(), kThreads, 0);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return 1;

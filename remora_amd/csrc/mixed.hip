@@ -23,7 +23,8 @@ constexpr int kT = 64;  // conversion tile
 __global__ __launch_bounds__(256) void convert_sym_kernel(const double* __restrict__ A, int64_t lda,
                                                           float* __restrict__ A32, int64_t ld32, int64_t n,
                                                           bool upper, double* __restrict__ rowsum) {
-    __shared__ double rs[kT], cs[kT];
+    __shared__ double mag[kT][kT + 1];  // |a| of the stored off-diagonal entries, [row][col]
+    __shared__ double rpart[4][kT], cpart[4][kT];
     // tile (ti, tj), ti >= tj, in the lower-tile enumeration
     const int64_t t = blockIdx.x;
     int64_t ti = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
@@ -33,27 +34,37 @@ __global__ __launch_bounds__(256) void convert_sym_kernel(const double* __restri
     // lower storage: rows ti, cols tj.  upper storage: rows tj, cols ti.
     const int64_t r0 = (upper ? tj : ti) * kT, c0 = (upper ? ti : tj) * kT;
     const int tid = threadIdx.x;
-    if (tid < kT) { rs[tid] = 0.0; cs[tid] = 0.0; }
-    __syncthreads();
     const int lr = tid & 63, lc0 = tid >> 6;  // 64 rows x 4 column phases
     double rsum = 0.0;
     const int64_t gi = r0 + lr;
     for (int lc = lc0; lc < kT; lc += 4) {
         const int64_t gj = c0 + lc;
-        if (gi >= n || gj >= n) continue;
-        const bool stored = upper ? (gi <= gj) : (gi >= gj);
-        if (!stored) continue;
-        const double v = A[gi + gj * lda];
-        A32[gi + gj * ld32] = (float)v;
-        const double a = fabs(v);
+        double a = 0.0;
+        if (gi < n && gj < n && (upper ? gi <= gj : gi >= gj)) {
+            const double v = A[gi + gj * lda];
+            A32[gi + gj * ld32] = (float)v;
+            a = fabs(v);
+        }
         rsum += a;
-        if (gi != gj) atomicAdd(&cs[lc], a);  // contributes to row gj too
+        mag[lr][lc] = gi != gj ? a : 0.0;  // an off-diagonal a_ij also counts for row j
     }
-    atomicAdd(&rs[lr], rsum);
+    rpart[lc0][lr] = rsum;
+    __syncthreads();
+    {  // column sums: thread (q, c) adds rows 16q .. 16q+15 of column c
+        const int c = tid & 63, q = tid >> 6;
+        double cs = 0.0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) cs += mag[16 * q + r][c];
+        cpart[q][c] = cs;
+    }
     __syncthreads();
     if (tid < kT) {
-        if (r0 + tid < n && rs[tid] != 0.0) atomicAdd(&rowsum[r0 + tid], rs[tid]);
-        if (c0 + tid < n && cs[tid] != 0.0) atomicAdd(&rowsum[c0 + tid], cs[tid]);
+        const double v = (rpart[0][tid] + rpart[1][tid]) + (rpart[2][tid] + rpart[3][tid]);
+        if (r0 + tid < n && v != 0.0) atomicAdd(&rowsum[r0 + tid], v);
+    } else if (tid < 2 * kT) {
+        const int c = tid - kT;
+        const double v = (cpart[0][c] + cpart[1][c]) + (cpart[2][c] + cpart[3][c]);
+        if (c0 + c < n && v != 0.0) atomicAdd(&rowsum[c0 + c], v);
     }
 }
 
