@@ -321,7 +321,10 @@ __device__ __forceinline__ void batched_posv_body(const BArgs<T>& a) {
             info = -1;
         } else if (n64 > 0) {
             const int n = (int)n64;
-            if (a.factor) info = INL ? factor_one_impl<T, NB>(a.A[mat], n, a.lda[mat]) : factor_one<T, NB>(a.A[mat], n, a.lda[mat]);
+            if (a.factor) {
+                if constexpr (INL) info = factor_one_impl<T, NB>(a.A[mat], n, a.lda[mat]);
+                else info = factor_one<T, NB>(a.A[mat], n, a.lda[mat]);
+            }
             if (info == 0 && a.nrhs > 0) {
                 if constexpr (INL) solve_one_impl<T, NB>(a.A[mat], n, a.lda[mat], a.B[mat], a.nrhs, a.ldb[mat]);
                 else solve_one<T, NB>(a.A[mat], n, a.lda[mat], a.B[mat], a.nrhs, a.ldb[mat]);
