@@ -127,10 +127,13 @@ __device__ __forceinline__ int64_t factor_one_impl(T* A, int n, int64_t lda) {
         }
         // rows below: x L^T = a, one row per thread; padded columns come out 0
         for (int r = tid; r < m; r += kThreads) {
-            T* row = A + (k + kb + r) + (int64_t)k * lda;
+            const T* row = A + (k + kb + r) + (int64_t)k * lda;
             double x[NB];
+            // unconditional loads from clamped columns; the padding is zeroed after
 #pragma unroll
-            for (int c = 0; c < NB; ++c) x[c] = (c < kb) ? (double)row[(int64_t)c * lda] : 0.0;
+            for (int c = 0; c < NB; ++c) x[c] = (double)row[(int64_t)min(c, kb - 1) * lda];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) x[c] = c < kb ? x[c] : 0.0;
 #pragma unroll
             for (int c = 0; c < NB; ++c) {
                 double sum[4] = {x[c], 0.0, 0.0, 0.0};
@@ -139,9 +142,14 @@ __device__ __forceinline__ int64_t factor_one_impl(T* A, int n, int64_t lda) {
                 x[c] = ((sum[0] + sum[1]) + (sum[2] + sum[3])) * s.rdiag[c];
                 __builtin_amdgcn_sched_barrier(0);  // bounds the LDS loads in flight (registers)
             }
+            // the store addresses are formed afresh: kept from the loads, the
+            // NB 64-bit addresses would stay live across the solve
+            int rr = r;
+            asm volatile("" : "+v"(rr));
+            T* wrow = A + (k + kb + rr) + (int64_t)k * lda;
 #pragma unroll
             for (int c = 0; c < NB; ++c) {
-                if (c < kb) row[(int64_t)c * lda] = (T)x[c];
+                if (c < kb) wrow[(int64_t)c * lda] = (T)x[c];
                 s.P[r * LD + c] = (T)x[c];
             }
         }
@@ -302,7 +310,7 @@ __device__ __attribute__((noinline)) void solve_one(const T* A, int n, int64_t l
 
 // The persistent loop.  FP64 calls the out-of-line routines (about 390
 // registers, one workgroup per CU).  FP32 inlines them into a kernel held
-// to 256 registers, two workgroups per CU: a few spills, but 11.2 against
+// to 256 registers, two workgroups per CU: a few spills, but 10.5 against
 // 13.3 ms on the 4096-system benchmark, where FP64 loses 7% the same way.
 template <typename T, int NB, bool INL>
 __device__ __forceinline__ void batched_posv_body(const BArgs<T>& a) {
