@@ -173,8 +173,9 @@ typedef struct rmh_batch_config {
  * (device pointers per system), d_order (optional processing order, a
  * permutation of 0..batch-1, NULL for 0..batch-1) -- so the launch can be
  * captured in a graph and replayed with new data.  d_info[i] receives 0, k+1
- * for a non-positive pivot k (X_i is then not computed), or -1 for n_i out of
- * range.  cfg NULL uses the handle's tuned configuration.  Asynchronous;
+ * for a non-positive pivot k (X_i is then not computed, and B_i holds the
+ * forward substitution of the columns before block column k), or -1 for n_i
+ * out of range (A_i and B_i untouched).  cfg NULL uses the handle's tuned configuration.  Asynchronous;
  * allocates nothing after the handle's first batched call.  No counterpart
  * in the reference (a new form beside the FP64 drop-in). */
 rmh_status rmh_dposv_batched(rmh_handle h, size_t batch, int max_n, const int64_t* d_n, double* const* d_A,

@@ -18,10 +18,13 @@
 //      LDS (P) as the operand of
 //   3. the trailing update A22 -= P P^T on the lower triangle, 16 x 16 MFMA
 //      tiles spread over the four waves, each wave prefetching its next tile.
-// The solve runs block by block: forward with L (diagonal block solved per
-// right-hand side, rows below updated one row per thread), then backward
-// with L^T (rows above updated one row per thread).  The factorisation and
-// the small solves are FP64; the trailing update is on MFMA in T.
+// The forward substitution with L runs inside the factorisation, on each
+// block column while its factor is in LDS (diagonal block solved per
+// right-hand side, rows below updated one row per thread); the backward
+// substitution with L^T follows (rows above updated one row per thread).  A
+// solve with a given factor runs both sweeps the same way.  The
+// factorisation and the small solves are FP64; the trailing update is on
+// MFMA in T.
 #include <algorithm>
 #include <vector>
 
@@ -96,10 +99,17 @@ struct Smem {
 template <typename T, int NB>
 __shared__ Smem<T, NB> g_bsm;
 
+template <int NB, int LD, int XL>
+__device__ __forceinline__ void diag_solve(const double* L, double* X, int nrhs, bool forward, int tid);
+
 // Factor the n x n matrix A (column-major, ld) in place.  Returns 0 or the
 // LAPACK info (k+1 of the first failing pivot).  Uniform across the block.
+// With nrhs > 0 it also runs the forward substitution L Y = B in place in B
+// (n x nrhs, ld ldb), block column by block column while the column's
+// factor is still in LDS, in the order and arithmetic of solve_one's
+// forward sweep.
 template <typename T, int NB>
-__device__ __forceinline__ int64_t factor_one_impl(T* A, int n, int64_t lda) {
+__device__ __forceinline__ int64_t factor_one_impl(T* A, int n, int64_t lda, T* B, int nrhs, int64_t ldb) {
     Smem<T, NB>& s = g_bsm<T, NB>;
     const int tid = threadIdx.x;
     typedef MfmaT<T> M;
@@ -154,6 +164,31 @@ __device__ __forceinline__ int64_t factor_one_impl(T* A, int n, int64_t lda) {
             }
         }
         __syncthreads();
+        if (nrhs > 0) {
+            // forward substitution of this block: Y_k = L_kk^-1 B_k, then
+            // B_rest -= L_rest,k Y_k with L_rest,k the rows in P
+            constexpr int XL = kMaxRhs + 1;
+            for (int e = tid; e < NB * nrhs; e += kThreads) {
+                const int i = e % NB, j = e / NB;
+                s.X[i * XL + j] = (i < kb) ? (double)B[(k + i) + (int64_t)j * ldb] : 0.0;
+            }
+            __syncthreads();
+            diag_solve<NB, LD, XL>(s.L, s.X, nrhs, true, tid);
+            __syncthreads();
+            for (int e = tid; e < kb * nrhs; e += kThreads) {
+                const int i = e % kb, j = e / kb;
+                B[(k + i) + (int64_t)j * ldb] = (T)s.X[i * XL + j];
+            }
+            for (int r = tid; r < m; r += kThreads) {
+                for (int j = 0; j < nrhs; ++j) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int c = 0; c < NB; ++c) acc = fma((double)s.P[r * LD + c], s.X[c * XL + j], acc);
+                    T* b = B + (k + kb + r) + (int64_t)j * ldb;
+                    *b = (T)((double)*b - acc);
+                }
+            }
+        }
         // trailing update of the lower triangle of A22 (m x m): each wave
         // takes groups of G tiles and issues all their C loads before the
         // MFMAs, so G tiles' worth of loads are in flight at once
@@ -226,9 +261,11 @@ __device__ __forceinline__ void diag_solve(const double* L, double* X, int nrhs,
     }
 }
 
-// Solve L L^T X = B in place (B column-major n x nrhs, ld ldb).
+// Solve L L^T X = B in place (B column-major n x nrhs, ld ldb); with
+// forward_done B already holds Y, L Y = B, and only L^T X = Y is solved.
 template <typename T, int NB>
-__device__ __forceinline__ void solve_one_impl(const T* A, int n, int64_t lda, T* B, int nrhs, int64_t ldb) {
+__device__ __forceinline__ void solve_one_impl(const T* A, int n, int64_t lda, T* B, int nrhs, int64_t ldb,
+                                               bool forward_done) {
     Smem<T, NB>& s = g_bsm<T, NB>;
     const int tid = threadIdx.x;
     constexpr int LD = Smem<T, NB>::LD, XL = kMaxRhs + 1;
@@ -243,7 +280,7 @@ __device__ __forceinline__ void solve_one_impl(const T* A, int n, int64_t lda, T
         }
     };
     // forward: L y = b
-    for (int k = 0; k < n; k += NB) {
+    for (int k = 0; k < n && !forward_done; k += NB) {
         const int kb = min(NB, n - k), m = n - k - kb;
         load_diag(k, kb);
         __syncthreads();
@@ -300,12 +337,13 @@ __device__ __forceinline__ void solve_one_impl(const T* A, int n, int64_t lda, T
 // Out-of-line copies: each routine then gets its own register allocation
 // and the persistent loop keeps almost nothing live across the calls.
 template <typename T, int NB>
-__device__ __attribute__((noinline)) int64_t factor_one(T* A, int n, int64_t lda) {
-    return factor_one_impl<T, NB>(A, n, lda);
+__device__ __attribute__((noinline)) int64_t factor_one(T* A, int n, int64_t lda, T* B, int nrhs, int64_t ldb) {
+    return factor_one_impl<T, NB>(A, n, lda, B, nrhs, ldb);
 }
 template <typename T, int NB>
-__device__ __attribute__((noinline)) void solve_one(const T* A, int n, int64_t lda, T* B, int nrhs, int64_t ldb) {
-    solve_one_impl<T, NB>(A, n, lda, B, nrhs, ldb);
+__device__ __attribute__((noinline)) void solve_one(const T* A, int n, int64_t lda, T* B, int nrhs, int64_t ldb,
+                                                    bool forward_done) {
+    solve_one_impl<T, NB>(A, n, lda, B, nrhs, ldb, forward_done);
 }
 
 // The persistent loop.  FP64 calls the out-of-line routines (about 390
@@ -330,12 +368,16 @@ __device__ __forceinline__ void batched_posv_body(const BArgs<T>& a) {
         } else if (n64 > 0) {
             const int n = (int)n64;
             if (a.factor) {
-                if constexpr (INL) info = factor_one_impl<T, NB>(a.A[mat], n, a.lda[mat]);
-                else info = factor_one<T, NB>(a.A[mat], n, a.lda[mat]);
+                // the forward substitution runs inside the factorisation
+                T* const b = a.nrhs > 0 ? a.B[mat] : nullptr;
+                const int64_t ldb = a.nrhs > 0 ? a.ldb[mat] : 1;
+                if constexpr (INL) info = factor_one_impl<T, NB>(a.A[mat], n, a.lda[mat], b, a.nrhs, ldb);
+                else info = factor_one<T, NB>(a.A[mat], n, a.lda[mat], b, a.nrhs, ldb);
             }
             if (info == 0 && a.nrhs > 0) {
-                if constexpr (INL) solve_one_impl<T, NB>(a.A[mat], n, a.lda[mat], a.B[mat], a.nrhs, a.ldb[mat]);
-                else solve_one<T, NB>(a.A[mat], n, a.lda[mat], a.B[mat], a.nrhs, a.ldb[mat]);
+                const bool fwd = a.factor != 0;
+                if constexpr (INL) solve_one_impl<T, NB>(a.A[mat], n, a.lda[mat], a.B[mat], a.nrhs, a.ldb[mat], fwd);
+                else solve_one<T, NB>(a.A[mat], n, a.lda[mat], a.B[mat], a.nrhs, a.ldb[mat], fwd);
             }
         }
         if (tid == 0 && a.info != nullptr) a.info[mat] = info;
