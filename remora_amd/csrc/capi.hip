@@ -129,8 +129,8 @@ Context::~Context() {
 
 // ------------------------------------------------------- TRSM driver --
 // Blocked recursive solve with inverted 64 x 64 diagonal blocks: the leaves
-// multiply by the inverse block (in place, LDS-staged), the off-diagonal
-// work is MFMA GEMM.  The split mirrors the reference's trsm_recursive
+// multiply by the inverse block in place, and they and the off-diagonal
+// work are MFMA GEMM.  The split mirrors the reference's trsm_recursive
 // (include/remora/kernels/default/trsm.hpp:227-261): halves in whole
 // blocks, lower solves front-to-back, upper back-to-front.
 namespace {
@@ -160,8 +160,28 @@ template <typename T>
 rmh_status trsm_rec(const TrsmPlan<T>& p, int64_t b0, int64_t b1) {
     const int64_t nb = kTrsmNb;
     if (b1 - b0 == 1) {
-        const int kb = (int)std::min(nb, p.na - b0 * nb);
-        return launch_tri_apply<T>(p.left, p.Dinv + b0 * nb * nb, nb, kb, p.B, p.ldb, b0 * nb, p.other, p.s);
+        // leaf: X_b = D B_b (left) or B_b D (right), D = inv(op(A)_bb), in
+        // place on the MFMA GEMM.  The product is one 64-row (left) or
+        // 64-column (right) tile high, so every workgroup reads the part of
+        // B it overwrites, and reads all of it before its epilogue writes.
+        const int64_t kb = std::min(nb, p.na - b0 * nb), r0 = b0 * nb;
+        const T* D = p.Dinv + b0 * nb * nb;
+        GemmDesc<T> g;
+        g.alpha = T(1);
+        g.beta = T(0);
+        g.K = kb;
+        if (p.left) {
+            g.M = kb; g.N = p.other;
+            g.A = D; g.lda = nb;
+            g.B = p.B + r0; g.ldb = p.ldb;
+            g.C = p.B + r0; g.ldc = p.ldb;
+        } else {
+            g.M = p.other; g.N = kb;
+            g.A = p.B + r0 * p.ldb; g.lda = p.ldb;
+            g.B = D; g.ldb = nb;
+            g.C = p.B + r0 * p.ldb; g.ldc = p.ldb;
+        }
+        return launch_gemm<T>(g, Tile::T64x64, p.s);
     }
     const int64_t mid = b0 + (b1 - b0) / 2;
     const int64_t r0 = b0 * nb, rm = mid * nb, r1 = std::min(b1 * nb, p.na);

@@ -88,45 +88,6 @@ __global__ __launch_bounds__(64) void trtri_strip_kernel(const T* A, int64_t lda
         for (int c = 0; c < kb; ++c) Dblk[lane + c * ldd] = (T)(lane >= c ? X[lane * kLdsLd + c] : 0.0);
 }
 
-// B_blk <- D * B_blk (left) / B_blk * D (right), kb <= 64, one workgroup per
-// 64-wide slab of the other dimension.
-template <typename T, bool LEFT>
-__global__ __launch_bounds__(256) void tri_apply_kernel(const T* D, int64_t ldd, int kb, T* B, int64_t ldb,
-                                                        int64_t r0, int64_t other) {
-    constexpr int S = 64;
-    __shared__ T Ds[kDiagMax * kLdsLd];
-    __shared__ T Bs[kDiagMax * kLdsLd];
-    const int tid = threadIdx.x;
-    const int64_t o0 = (int64_t)blockIdx.x * S;
-    const int ow = (int)min((int64_t)S, other - o0);
-    for (int e = tid; e < kb * kb; e += 256) {
-        const int i = e % kb, j = e / kb;
-        Ds[i * kLdsLd + j] = D[i + (int64_t)j * ldd];
-    }
-    // slab element (t, o): LEFT: B[(r0+t) + (o0+o)*ldb]  RIGHT: B[(o0+o) + (r0+t)*ldb]
-    for (int e = tid; e < kb * ow; e += 256) {
-        int t, o;
-        if (LEFT) { t = e % kb; o = e / kb; }
-        else { o = e % ow; t = e / ow; }
-        const int64_t idx = LEFT ? (r0 + t) + (o0 + o) * ldb : (o0 + o) + (r0 + t) * ldb;
-        Bs[t * kLdsLd + o] = B[idx];
-    }
-    __syncthreads();
-    for (int e = tid; e < kb * ow; e += 256) {
-        int t, o;
-        if (LEFT) { t = e % kb; o = e / kb; }
-        else { o = e % ow; t = e / ow; }
-        double acc = 0.0;
-        if (LEFT) {  // (D * Bblk)(t, o) = sum_u D(t,u) Bblk(u,o)
-            for (int u = 0; u < kb; ++u) acc = fma((double)Ds[t * kLdsLd + u], (double)Bs[u * kLdsLd + o], acc);
-        } else {     // (Bblk * D)(o, t) = sum_u Bblk(o,u) D(u,t); slab stored as Bs[u][o]
-            for (int u = 0; u < kb; ++u) acc = fma((double)Bs[u * kLdsLd + o], (double)Ds[u * kLdsLd + t], acc);
-        }
-        const int64_t idx = LEFT ? (r0 + t) + (o0 + o) * ldb : (o0 + o) + (r0 + t) * ldb;
-        B[idx] = (T)acc;
-    }
-}
-
 template <typename T>
 __global__ void scale_kernel(T* B, int64_t ldb, int64_t rows, int64_t cols, T alpha) {
     const int64_t total = rows * cols;
@@ -215,19 +176,6 @@ rmh_status launch_trtri_strip(const T* A, int64_t lda, int64_t n, bool s_is_At, 
 }
 
 template <typename T>
-rmh_status launch_tri_apply(bool left, const T* Dinv, int64_t ldd, int kb, T* B, int64_t ldb, int64_t r0,
-                            int64_t other, hipStream_t stream) {
-    if (kb <= 0 || other <= 0) return RMH_STATUS_SUCCESS;
-    if (kb > kDiagMax) return fail(RMH_STATUS_INVALID_VALUE, "tri_apply: block larger than 64");
-    const unsigned g = (unsigned)((other + 63) / 64);
-    if (left)
-        hipLaunchKernelGGL((tri_apply_kernel<T, true>), dim3(g), dim3(256), 0, stream, Dinv, ldd, kb, B, ldb, r0, other);
-    else
-        hipLaunchKernelGGL((tri_apply_kernel<T, false>), dim3(g), dim3(256), 0, stream, Dinv, ldd, kb, B, ldb, r0, other);
-    return check_launch("tri_apply_kernel");
-}
-
-template <typename T>
 rmh_status launch_scale(T* B, int64_t ldb, int64_t rows, int64_t cols, T alpha, hipStream_t stream) {
     if (rows <= 0 || cols <= 0) return RMH_STATUS_SUCCESS;
     hipLaunchKernelGGL(scale_kernel<T>, dim3(grid_for(rows * cols, 256)), dim3(256), 0, stream, B, ldb, rows, cols, alpha);
@@ -265,7 +213,6 @@ rmh_status launch_fill_i64(int64_t* p, int64_t n, int64_t value, hipStream_t str
 #define RMH_INST(T)                                                                                         \
     template rmh_status launch_trtri_blocks<T>(const T*, int64_t, int64_t, int, bool, bool, bool, T*,       \
                                                hipStream_t);                                               \
-    template rmh_status launch_tri_apply<T>(bool, const T*, int64_t, int, T*, int64_t, int64_t, int64_t, hipStream_t); \
     template rmh_status launch_scale<T>(T*, int64_t, int64_t, int64_t, T, hipStream_t);                    \
     template rmh_status launch_trtri_strip<T>(const T*, int64_t, int64_t, bool, bool, T*, int64_t, int64_t,  \
                                               hipStream_t);                                               \

@@ -118,14 +118,6 @@ template <typename T>
 rmh_status launch_trtri_blocks(const T* A, int64_t lda, int64_t n, int nb, bool upper, bool trans, bool unit,
                                T* Dinv, hipStream_t stream);
 
-// In-place B_blk <- Dinv_b * B_blk (left) or B_blk <- B_blk * Dinv_b
-// (right); for left, B_blk is rows [r0, r0+kb) of the m x nrhs block of B,
-// for right, B_blk is columns [r0, r0+kb).  `other` is the extent of the
-// non-blocked dimension.
-template <typename T>
-rmh_status launch_tri_apply(bool left, const T* Dinv, int64_t ldd, int kb, T* B, int64_t ldb, int64_t r0,
-                            int64_t other, hipStream_t stream);
-
 // Inverses of the 64-blocks of the lower-triangular S = (s_is_At ? A^T : A)
 // into a block-diagonal strip D (n x nbd, ld ldd): element (i, j) of the
 // inverse of an nbd-block lives at D[i + (j mod nbd)*ldd].
