@@ -9,10 +9,12 @@
 // Phi is self-adjoint.)  Upper storage (A = U^T U) is the same with L = U^T,
 // Lbar = Ubar^T.
 //
-// Device steps: zero-padded lower copies of L and Lbar; P on the MFMA GEMM
-// (lower part only, tiles past the diagonal skipped); Phi; the two solves on
-// the blocked TRSM, which rebuilds the inverses of L's diagonal blocks each
-// call instead of storing L^-1; an in-place symmetrisation.  No counterpart
+// Device steps: zero-padded copies U = L^T (upper) and tril(Lbar); P = U Lbar
+// on the MFMA GEMM (lower part only, each tile skipping the depths below its
+// first row; U as the plain, untransposed operand); Phi; the two solves
+// with U (upper storage) on the TRSM, which rebuilds the inverses of the
+// diagonal blocks each call instead of storing L^-1; an in-place
+// symmetrisation.  No counterpart
 // in the reference (a new form beside the FP64 drop-in).
 #include "rmh_context.hpp"
 
@@ -51,6 +53,40 @@ __global__ __launch_bounds__(256) void tril_copy_kernel(const T* __restrict__ sr
     for (int r = ty; r < kT; r += 8) {
         const int64_t i = i0 + tx, j = j0 + r;
         if (i < n && j < n) dst[i + j * ldd] = (i >= j) ? tile[tx][r] : T(0);
+    }
+}
+
+// dst = tril(op(src))^T, an upper triangular matrix with zeros below:
+// dst(i, j) = src(j, i) (lower storage) or src(i, j) (upper storage), i <= j
+template <typename T>
+__global__ __launch_bounds__(256) void triu_t_copy_kernel(const T* __restrict__ src, int64_t lds, int64_t n, bool upper,
+                                                          T* __restrict__ dst, int64_t ldd) {
+    __shared__ T tile[kT][kT + 1];
+    const int64_t i0 = (int64_t)blockIdx.x * kT, j0 = (int64_t)blockIdx.y * kT;
+    const int tx = threadIdx.x % kT, ty = threadIdx.x / kT;  // 32 x 8
+    if (i0 > j0) {  // strictly below the diagonal: zeros
+        for (int r = ty; r < kT; r += 8) {
+            const int64_t i = i0 + tx, j = j0 + r;
+            if (i < n && j < n) dst[i + j * ldd] = T(0);
+        }
+        return;
+    }
+    if (upper) {
+        for (int r = ty; r < kT; r += 8) {
+            const int64_t i = i0 + tx, j = j0 + r;
+            if (i < n && j < n) dst[i + j * ldd] = (i <= j) ? src[i + j * lds] : T(0);
+        }
+        return;
+    }
+    // lower: dst(i, j) = src(j, i); read the transposed tile coalesced
+    for (int r = ty; r < kT; r += 8) {
+        const int64_t si = j0 + tx, sj = i0 + r;  // src row, col
+        tile[r][tx] = (si < n && sj < n) ? src[si + sj * lds] : T(0);
+    }
+    __syncthreads();
+    for (int r = ty; r < kT; r += 8) {
+        const int64_t i = i0 + tx, j = j0 + r;
+        if (i < n && j < n) dst[i + j * ldd] = (i <= j) ? tile[tx][r] : T(0);
     }
 }
 
@@ -101,19 +137,20 @@ rmh_status potrf_adjoint_impl(Context* h, bool upper, int64_t n, const T* L, int
     const size_t mat = (size_t)n * n * sizeof(T);
     void* w = nullptr;
     RMH_TRY(h->reserve_slot(WS_ADJOINT, 2 * ((mat + 255) / 256 * 256), &w));
-    T* Lz = static_cast<T*>(w);
+    T* U = static_cast<T*>(w);
     T* Lbz = reinterpret_cast<T*>(static_cast<char*>(w) + (mat + 255) / 256 * 256);
     const unsigned nt = (unsigned)((n + kT - 1) / kT);
-    hipLaunchKernelGGL((tril_copy_kernel<T>), dim3(nt, nt), dim3(256), 0, s, L, ldl, n, upper, Lz, n);
-    RMH_TRY(check_launch("tril_copy_kernel"));
+    hipLaunchKernelGGL((triu_t_copy_kernel<T>), dim3(nt, nt), dim3(256), 0, s, L, ldl, n, upper, U, n);
+    RMH_TRY(check_launch("triu_t_copy_kernel"));
     hipLaunchKernelGGL((tril_copy_kernel<T>), dim3(nt, nt), dim3(256), 0, s, Lbar, ldlb, n, upper, Lbz, n);
     RMH_TRY(check_launch("tril_copy_kernel"));
-    // P = lower(L^T Lbar) into Abar.  Both factors are lower triangular, so
-    // for a tile whose rows start at r0 only k >= r0 contributes.
+    // P = lower(U Lbar), U = L^T, into Abar.  U is upper and Lbar lower
+    // triangular, so for a tile whose rows start at r0 only k >= r0
+    // contributes.
     GemmDesc<T> g;
     g.M = n; g.N = n; g.K = n;
     g.alpha = T(1); g.beta = T(0);
-    g.A = Lz; g.lda = n; g.transA = true;
+    g.A = U; g.lda = n; g.transA = false;
     g.B = Lbz; g.ldb = n; g.transB = false;
     g.C = Abar; g.ldc = ldab;
     g.part = Part::Lower;
@@ -121,9 +158,9 @@ rmh_status potrf_adjoint_impl(Context* h, bool upper, int64_t n, const T* L, int
     RMH_TRY(launch_gemm<T>(g, pick_tile(n, n, Part::Lower), s));
     hipLaunchKernelGGL((phi_kernel<T>), dim3(nt, nt), dim3(256), 0, s, Abar, ldab, n);
     RMH_TRY(check_launch("phi_kernel"));
-    // G = L^-T P L^-1
-    RMH_TRY(trsm_impl<T>(h, true, false, true, false, n, n, T(1), Lz, n, Abar, ldab, s));
-    RMH_TRY(trsm_impl<T>(h, false, false, false, false, n, n, T(1), Lz, n, Abar, ldab, s));
+    // G = L^-T P L^-1: U X = P, then X U^T = (.)
+    RMH_TRY(trsm_impl<T>(h, true, true, false, false, n, n, T(1), U, n, Abar, ldab, s));
+    RMH_TRY(trsm_impl<T>(h, false, true, true, false, n, n, T(1), U, n, Abar, ldab, s));
     hipLaunchKernelGGL((symmetrize_kernel<T>), dim3(nt, nt), dim3(256), 0, s, Abar, ldab, n);
     return check_launch("symmetrize_kernel");
 }

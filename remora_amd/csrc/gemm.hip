@@ -172,9 +172,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     }
 
     // ---- tile selection: XCD-contiguous ranges, then grouped order -------
+    // Workgroups go to the XCDs in turn; each XCD takes a contiguous range
+    // of tiles so that neighbouring tiles share its L2.  With k_from_row the
+    // depth shrinks row by row, so contiguous ranges would give the first
+    // XCD several times the work of the last: those tiles stay in row order,
+    // dealt out to the XCDs in turn.
     const int64_t G = gridDim.x;
     int64_t pid = blockIdx.x;
-    {
+    if (!p.k_from_row) {
         const int64_t full = (G / kXcd) * kXcd;
         if (pid < full) pid = (pid % kXcd) * (G / kXcd) + pid / kXcd;
     }
