@@ -145,8 +145,14 @@ struct TrsmPlan {
     T* B;
     int64_t ldb;
     int64_t na, other;
-    const T* Dinv;
+    const T* Dinv;  // nb = 64: inv(op(A)_bb) per block, contiguous, ld 64
     hipStream_t s;
+    int64_t nb = kTrsmNb;
+    // nb > 64: D holds inv(S_bb) as an na x nb strip (ld na), S the lower
+    // triangular A or A^T, and tmp (nb x other or other x nb) takes the
+    // leaf product before it is copied back
+    const T* D = nullptr;
+    T* tmp = nullptr;
 };
 
 // Block T[r0:r1, c0:c1] of op(A) as a GEMM operand.
@@ -158,7 +164,37 @@ void tri_operand(const TrsmPlan<T>& p, int64_t r0, int64_t c0, const T** ptr, bo
 
 template <typename T>
 rmh_status trsm_rec(const TrsmPlan<T>& p, int64_t b0, int64_t b1) {
-    const int64_t nb = kTrsmNb;
+    const int64_t nb = p.nb;
+    if (b1 - b0 == 1 && nb > kTrsmNb) {
+        // wide leaf: X_b = inv(op(A)_bb) B_b (left) or B_b inv(op(A)_bb)
+        // (right) into tmp, then back into B; inv(op(A)_bb) = Db or Db^T
+        const int64_t kb = std::min(nb, p.na - b0 * nb), r0 = b0 * nb;
+        const T* Db = p.D + r0;
+        GemmDesc<T> g;
+        g.alpha = T(1);
+        g.beta = T(0);
+        g.K = kb;
+        if (p.left) {
+            g.M = kb; g.N = p.other;
+            g.A = Db; g.lda = p.na; g.transA = p.eff_upper;
+            g.B = p.B + r0; g.ldb = p.ldb;
+            g.C = p.tmp; g.ldc = kb;
+        } else {
+            g.M = p.other; g.N = kb;
+            g.A = p.B + r0 * p.ldb; g.lda = p.ldb;
+            g.B = Db; g.ldb = p.na; g.transB = p.eff_upper;
+            g.C = p.tmp; g.ldc = p.other;
+        }
+        RMH_TRY(launch_gemm<T>(g, pick_tile(g.M, g.N, Part::Full), p.s));
+        if (p.left)
+            RMH_TRY_HIP(hipMemcpy2DAsync(p.B + r0, (size_t)p.ldb * sizeof(T), p.tmp, (size_t)kb * sizeof(T),
+                                         (size_t)kb * sizeof(T), (size_t)p.other, hipMemcpyDeviceToDevice, p.s));
+        else
+            RMH_TRY_HIP(hipMemcpy2DAsync(p.B + r0 * p.ldb, (size_t)p.ldb * sizeof(T), p.tmp,
+                                         (size_t)p.other * sizeof(T), (size_t)p.other * sizeof(T), (size_t)kb,
+                                         hipMemcpyDeviceToDevice, p.s));
+        return RMH_STATUS_SUCCESS;
+    }
     if (b1 - b0 == 1) {
         // leaf: X_b = D B_b (left) or B_b D (right), D = inv(op(A)_bb), in
         // place on the MFMA GEMM.  The product is one 64-row (left) or
@@ -233,12 +269,28 @@ rmh_status trsm_impl(Context* h, bool left, bool upper, bool trans, bool unit, i
         return trsm_blocked<T>(h, left, upper, trans, unit, m, n, alpha, A, lda, B, ldb, s);
     if (alpha != T(1)) RMH_TRY(launch_scale<T>(B, ldb, m, n, alpha, s));
     if (alpha == T(0)) return RMH_STATUS_SUCCESS;
-    const int64_t na = left ? m : n;
+    const int64_t na = left ? m : n, other = left ? n : m;
+    if (na >= 2048 && other >= 512) {
+        // many right-hand sides: 512-wide leaves from block inverses built
+        // by doubling, so the recursion stops before its GEMMs get thin
+        constexpr int64_t nbd = 512;
+        auto up = [](size_t e) { return (e + 63) / 64 * 64; };
+        const size_t d_el = up((size_t)na * nbd), w_el = up((size_t)na * nbd / 2 + 64 * 64),
+                     t_el = up((size_t)nbd * other);
+        RMH_TRY(h->reserve((d_el + w_el + t_el) * sizeof(T)));
+        T* D = static_cast<T*>(h->ws);
+        RMH_TRY(diag_block_inverses<T>(A, lda, na, upper, unit, nbd, D, D + d_el, s));
+        TrsmPlan<T> p{left, upper != trans, trans, A, lda, B, ldb, na, other, nullptr, s};
+        p.nb = nbd;
+        p.D = D;
+        p.tmp = D + d_el + w_el;
+        return trsm_rec(p, 0, (na + nbd - 1) / nbd);
+    }
     const int64_t nblk = (na + kTrsmNb - 1) / kTrsmNb;
     RMH_TRY(h->reserve((size_t)nblk * kTrsmNb * kTrsmNb * sizeof(T)));
     T* Dinv = static_cast<T*>(h->ws);
     RMH_TRY(launch_trtri_blocks<T>(A, lda, na, kTrsmNb, upper, trans, unit, Dinv, s));
-    TrsmPlan<T> p{left, upper != trans, trans, A, lda, B, ldb, na, left ? n : m, Dinv, s};
+    TrsmPlan<T> p{left, upper != trans, trans, A, lda, B, ldb, na, other, Dinv, s};
     return trsm_rec(p, 0, nblk);
 }
 

@@ -76,6 +76,12 @@ struct TrsmInverses {
     T* D = nullptr;  // na x nbd strip of block inverses (ld na)
     T* P = nullptr;  // split-K partials
 };
+// D (na x nbd strip, ld na) <- inverses of the nbd x nbd diagonal blocks of
+// S = A (lower storage) or A^T (upper storage); nbd = 2^k >= 64; W needs
+// na * nbd / 2 + 64 * 64 elements.
+template <typename T>
+rmh_status diag_block_inverses(const T* A, int64_t lda, int64_t na, bool upper, bool unit, int64_t nbd, T* D, T* W,
+                               hipStream_t s);
 // bytes of workspace trsm_prepare needs for order na and `other` right-hand sides
 template <typename T>
 size_t trsm_inverse_workspace(int64_t na, int64_t other);

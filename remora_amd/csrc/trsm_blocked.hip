@@ -139,6 +139,14 @@ size_t trsm_inverse_workspace(int64_t na, int64_t other) {
 }
 
 template <typename T>
+rmh_status diag_block_inverses(const T* A, int64_t lda, int64_t na, bool upper, bool unit, int64_t nbd, T* D, T* W,
+                               hipStream_t s) {
+    if (nbd < 64 || (nbd & (nbd - 1)) != 0) return fail(RMH_STATUS_INVALID_VALUE, "block inverses: order not 2^k >= 64");
+    const SView<T> S{A, lda, upper};
+    return build_block_inverses<T>(S, na, unit, nbd, D, W, s);
+}
+
+template <typename T>
 rmh_status trsm_prepare(const T* A, int64_t lda, int64_t na, bool upper, bool unit, void* ws, TrsmInverses<T>* inv,
                         hipStream_t s) {
     const int64_t nbd = choose_nbd<T>(na);
@@ -258,5 +266,10 @@ template rmh_status trsm_blocked<double>(Context*, bool, bool, bool, bool, int64
                                          int64_t, double*, int64_t, hipStream_t);
 template rmh_status trsm_blocked<float>(Context*, bool, bool, bool, bool, int64_t, int64_t, float, const float*,
                                         int64_t, float*, int64_t, hipStream_t);
+
+template rmh_status diag_block_inverses<double>(const double*, int64_t, int64_t, bool, bool, int64_t, double*, double*,
+                                                hipStream_t);
+template rmh_status diag_block_inverses<float>(const float*, int64_t, int64_t, bool, bool, int64_t, float*, float*,
+                                               hipStream_t);
 
 }  // namespace rmh

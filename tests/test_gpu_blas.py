@@ -267,3 +267,24 @@ def test_strsm_large_blocks(handle, left, upper, trans):
     resid = (opT @ X - B) if left else (X @ opT - B)
     scale = (np.abs(opT) @ np.abs(X)) if left else (np.abs(X) @ np.abs(opT))
     assert np.max(np.abs(resid) / scale) < 1e-4
+
+
+@pytest.mark.parametrize("left,upper,trans,unit", CASES[::2])
+def test_dtrsm_many_rhs_wide_leaves(handle, left, upper, trans, unit):
+    """Order 2100 with 600 right-hand sides: the recursive solve with
+    512-wide leaves (block inverses built by doubling, a 52-row last block)."""
+    from remora_amd import linalg as la
+    n, other = 2100, 600
+    L = synth.uniform(7, 0, n * n, -1, 1).reshape(n, n) / n + np.eye(n) * 2.0
+    A = np.tril(L) if not upper else np.triu(L)
+    B = synth.uniform(7, 1, n * other, -1, 1).reshape((n, other) if left else (other, n))
+    dB = dev(B)
+    la.trsm(handle, dev(A), dB, left=left, upper=upper, trans=trans, unit=unit, alpha=0.5)
+    X = host(dB)
+    T = A.copy()
+    if unit:
+        T = T - np.diag(np.diag(T)) + np.eye(n)
+    opT = T.T if trans else T
+    resid = (opT @ X - 0.5 * B) if left else (X @ opT - 0.5 * B)
+    scale = (np.abs(opT) @ np.abs(X)) if left else (np.abs(X) @ np.abs(opT))
+    assert np.max(np.abs(resid) / scale) < 1e-12
