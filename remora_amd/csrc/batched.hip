@@ -88,6 +88,7 @@ struct Smem {
     double L[NB * LD];        // diagonal block / its factor
     double rdiag[NB];
     T P[kMaxN * LD];          // solved rows below the diagonal block
+    T P2[NB == 16 ? kMaxN * LD : 1];  // NB 16: the previous block's rows (deferred update)
     double X[NB * (kMaxRhs + 1)];  // solution block of the solve
     int bad;
     int64_t item;
@@ -118,8 +119,44 @@ __device__ __forceinline__ int64_t factor_one_impl(T* A, int n, int64_t lda, T* 
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int li = lane & 15, lk = lane >> 4;
+    // NB 16 defers every other block's trailing update: block k's rows stay
+    // in LDS, block k+16's column takes their update alone, and after block
+    // k+16 is factored one rank-32 pass updates the rest with both -- the
+    // trailing triangle is read and written half as often.
+    int cur = 0;           // buffer holding this block's rows (0: P, 1: P2)
+    bool pending = false;  // the other buffer holds the previous block's rows, not yet applied
+    auto Pbuf = [&](int b, int idx) -> T& { return (NB == 16 && b) ? s.P2[idx] : s.P[idx]; };
     for (int k = 0; k < n; k += NB) {
         const int kb = min(NB, n - k), m = n - k - kb;
+        if (NB == 16 && pending) {
+            // this block column (rows k.., its kb columns) takes the previous
+            // block's update: tiles (ti, 0), K = 16.  The previous block's row
+            // r is matrix row k + r.
+            const int prv = cur ^ 1;
+            const int rows = n - k, tm = (rows + 15) / 16;
+            T* Ak = A + k + (int64_t)k * lda;
+            for (int t = wave; t < tm; t += 4) {
+                T c4[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = 16 * t + M::row(lane, r), j = li;
+                    c4[r] = (i < rows && j < kb && i >= j) ? Ak[i + (int64_t)j * lda] : T(0);
+                }
+                acc_t acc = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const T a = Pbuf(prv, (16 * t + li) * LD + 4 * q + lk);
+                    const T b = Pbuf(prv, li * LD + 4 * q + lk);
+                    acc = M::op(a, b, acc);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = 16 * t + M::row(lane, r), j = li;
+                    if (i < rows && j < kb && i >= j) Ak[i + (int64_t)j * lda] = c4[r] - acc[r];
+                }
+            }
+            __syncthreads();
+        }
         for (int e = tid; e < NB * NB; e += kThreads) {
             const int i = e % NB, c = e / NB;
             s.L[i * LD + c] = (i < kb && c <= i) ? (double)A[(k + i) + (int64_t)(k + c) * lda] : (i == c ? 1.0 : 0.0);
@@ -160,7 +197,7 @@ __device__ __forceinline__ int64_t factor_one_impl(T* A, int n, int64_t lda, T* 
 #pragma unroll
             for (int c = 0; c < NB; ++c) {
                 if (c < kb) wrow[(int64_t)c * lda] = (T)x[c];
-                s.P[r * LD + c] = (T)x[c];
+                Pbuf(cur, r * LD + c) = (T)x[c];
             }
         }
         __syncthreads();
@@ -183,15 +220,26 @@ __device__ __forceinline__ int64_t factor_one_impl(T* A, int n, int64_t lda, T* 
                 for (int j = 0; j < nrhs; ++j) {
                     double acc = 0.0;
 #pragma unroll
-                    for (int c = 0; c < NB; ++c) acc = fma((double)s.P[r * LD + c], s.X[c * XL + j], acc);
+                    for (int c = 0; c < NB; ++c) acc = fma((double)Pbuf(cur, r * LD + c), s.X[c * XL + j], acc);
                     T* b = B + (k + kb + r) + (int64_t)j * ldb;
                     *b = (T)((double)*b - acc);
                 }
             }
         }
+        if (NB == 16 && !pending && m > 0) {
+            // defer: the next block column takes this block's update first
+            pending = true;
+            cur ^= 1;
+            __syncthreads();
+            continue;
+        }
         // trailing update of the lower triangle of A22 (m x m): each wave
         // takes groups of G tiles and issues all their C loads before the
-        // MFMAs, so G tiles' worth of loads are in flight at once
+        // MFMAs, so G tiles' worth of loads are in flight at once; with a
+        // deferred previous block its rows from 16 on (matrix rows k + 16 ..)
+        // add a second rank-16 term
+        const bool two = NB == 16 && pending;
+        const int prv = cur ^ 1;
         constexpr int G = 8;
         const int tm = (m + 15) / 16, ntiles = tm * (tm + 1) / 2;
         T* A22 = A + (k + kb) + (int64_t)(k + kb) * lda;
@@ -213,9 +261,17 @@ __device__ __forceinline__ int64_t factor_one_impl(T* A, int n, int64_t lda, T* 
                 acc_t acc = {T(0), T(0), T(0), T(0)};
 #pragma unroll
                 for (int q = 0; q < NB / 4; ++q) {
-                    const T a = s.P[(16 * ti[g] + li) * LD + 4 * q + lk];
-                    const T b = s.P[(16 * tj[g] + li) * LD + 4 * q + lk];
+                    const T a = Pbuf(cur, (16 * ti[g] + li) * LD + 4 * q + lk);
+                    const T b = Pbuf(cur, (16 * tj[g] + li) * LD + 4 * q + lk);
                     acc = M::op(a, b, acc);
+                }
+                if (two) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const T a = Pbuf(prv, (16 + 16 * ti[g] + li) * LD + 4 * q + lk);
+                        const T b = Pbuf(prv, (16 + 16 * tj[g] + li) * LD + 4 * q + lk);
+                        acc = M::op(a, b, acc);
+                    }
                 }
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
@@ -224,6 +280,7 @@ __device__ __forceinline__ int64_t factor_one_impl(T* A, int n, int64_t lda, T* 
                 }
             }
         }
+        pending = false;
         __syncthreads();
     }
     return 0;
