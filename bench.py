@@ -39,6 +39,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"f64": 78.6e12, "f32": 157.3e12}
+HBM_PEAK = 8.0e12
 TOL = 1e-14
 
 
@@ -301,6 +302,46 @@ def run_adjoint(h, steps, warmup, n=8192):
                          "frac": achieved / PEAK["f64"]}}
 
 
+def run_update(h, steps, warmup, n=16384):
+    """Rank-one update of a Cholesky factor (cholesky_decomposition::update)
+    at n=16384, FP64, lower storage: L tril(N(0,1))/sqrt(n) + 2 I (seed 21),
+    v N(0,1)/sqrt(n), alpha 1, beta 0.5.  The factor is restored from a copy
+    before each step (outside the timed events).  Algorithmic traffic: the
+    stored triangle read once and written once, n(n+1) * 8 bytes (+ v)."""
+    import torch
+    from remora_amd import linalg as la
+    g = torch.Generator(device="cuda").manual_seed(21)
+    L0 = la.colmajor(torch.tril(torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)) / n ** 0.5
+                     + 2.0 * torch.eye(n, dtype=torch.float64, device="cuda"))
+    v = torch.randn(n, dtype=torch.float64, device="cuda", generator=g) / n ** 0.5
+    L = la.empty_colmajor(n, n)
+    info = torch.zeros(1, dtype=torch.int64, device="cuda")
+    times = []
+    for i in range(warmup + steps):
+        L.copy_(L0)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        la.chol_update(h, L, 1.0, 0.5, v, info=info)
+        e1.record()
+        torch.cuda.synchronize()
+        assert int(info.item()) == 0
+        if i >= warmup:
+            times.append(e0.elapsed_time(e1))
+    # sampled invariant: (L' L'^T)_ij = (L L^T)_ij + 0.5 v_i v_j
+    idx = torch.randint(0, n, (2, 256), generator=g, device="cuda")
+    Ln, Lo = torch.tril(L), torch.tril(L0)
+    lhs = (Ln[idx[0]] * Ln[idx[1]]).sum(1)
+    rhs = (Lo[idx[0]] * Lo[idx[1]]).sum(1) + 0.5 * v[idx[0]] * v[idx[1]]
+    err = float(((lhs - rhs).abs().max() / rhs.abs().max()).item())
+    ms = median(times)
+    nbytes = float(n * (n + 1) * 8 + 2 * n * 8)
+    achieved = nbytes / (ms * 1e-3)
+    return {"gbps": achieved / 1e9, "ms": ms, "step_ms": times, "n": n, "algorithmic_bytes": nbytes,
+            "accuracy": {"max_rel_err_sampled_invariant": err},
+            "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK}}
+
+
 def median(v):
     s = sorted(v)
     return s[len(s) // 2]
@@ -364,6 +405,7 @@ def main():
     ap.add_argument("--only-fp64", action="store_true", help="FP64 legs only (no mixed leg)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-systems legs")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the Cholesky-adjoint leg")
+    ap.add_argument("--no-update", action="store_true", help="skip the rank-one update leg")
     ap.add_argument("--nb", type=int, default=0, help="potrf panel width (0: library default)")
     ap.add_argument("--nbi", type=int, default=0, help="potrf diagonal block (0: library default)")
     ap.add_argument("--no-lookahead", action="store_true", help="single-stream factorisation")
@@ -462,6 +504,14 @@ def main():
         paths["adjoint_f64"] = r
         log(f"[bench] adjoint: median {r['ms']:.3f} ms  {r['tflops']:.2f} TFLOP/s  "
             f"err vs closed form {r['accuracy']['max_rel_err_vs_closed_form']:.2e}")
+        torch.cuda.empty_cache()
+
+    if not args.no_update:
+        log("[bench] rank-one Cholesky update, n=16384")
+        r = run_update(h, args.steps, args.warmup)
+        paths["chol_update_f64"] = r
+        log(f"[bench] update: median {r['ms']:.3f} ms  {r['gbps']:.0f} GB/s "
+            f"({100 * r['roofline']['frac']:.1f}% of HBM)  invariant err {r['accuracy']['max_rel_err_sampled_invariant']:.2e}")
         torch.cuda.empty_cache()
 
     cpu = None

@@ -116,6 +116,21 @@ rmh_status rmh_dpotrs(rmh_handle h, int upper, size_t n, size_t nrhs, const doub
 rmh_status rmh_spotrs(rmh_handle h, int upper, size_t n, size_t nrhs, const float* A, size_t lda, float* B,
                       size_t ldb, rmh_stream_t stream);
 
+/* Rank-one update of a Cholesky factor in place: with A = L L^T on entry
+ * (the factor from rmh_?potrf: L lower, or U = L^T upper; only that triangle
+ * read and written), on return L L^T = alpha*A + beta*v*v^T.  Replaces
+ * cholesky_decomposition::update (include/remora/decompositions.hpp:132-169),
+ * same arguments (alpha >= 0; beta of either sign).  v: n elements at stride
+ * incv, unchanged.  beta == 0 scales the whole n x n stored matrix by
+ * sqrt(alpha), as the reference does.  d_info (device int64, not NULL)
+ * receives 0, or j+1 when the update would make the matrix indefinite at
+ * column j (the reference's std::invalid_argument); L is then unchanged.
+ * Asynchronous: read d_info after the stream. */
+rmh_status rmh_dchol_update(rmh_handle h, int upper, size_t n, double alpha, double beta, double* L, size_t ldl,
+                            const double* v, size_t incv, int64_t* d_info, rmh_stream_t stream);
+rmh_status rmh_schol_update(rmh_handle h, int upper, size_t n, float alpha, float beta, float* L, size_t ldl,
+                            const float* v, size_t incv, int64_t* d_info, rmh_stream_t stream);
+
 /* ----------------------------------------------------- mixed precision -- */
 /* Precision of the factorisation inside rmh_dsposv. */
 typedef enum rmh_factor_precision {

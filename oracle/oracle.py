@@ -47,6 +47,8 @@ def _declare(lib):
     lib.orc_dgemm.restype = None
     lib.orc_dposv.argtypes = [P, S, S, I, P, S, S, S, I, I, C.POINTER(S)]
     lib.orc_dposv.restype = I
+    lib.orc_dchol_update.argtypes = [P, S, S, I, D, D, P, S]
+    lib.orc_dchol_update.restype = S
     lib.orc_vector_length.restype = I
     lib.orc_simd_enabled.restype = I
 
@@ -133,6 +135,23 @@ def posv(A, B, left: bool = True, a_row_major=True, b_row_major=True, lib=None):
     sing = lib.orc_dposv(_ptr(a), n, max(_ld(a, a_row_major), 1), int(a_row_major), _ptr(b), b.shape[0],
                          b.shape[1], max(_ld(b, b_row_major), 1), int(b_row_major), int(left), C.byref(info))
     return b, a, int(info.value), bool(sing)
+
+
+def chol_update(L, alpha: float, beta: float, v, row_major: bool = True, incv: int = 1, lib=None):
+    """cholesky_decomposition::update(alpha, beta, v) on a lower factor
+    (decompositions.hpp:132-169).  v holds n elements at stride incv.
+
+    Returns (updated copy of L, info): info 0, or j+1 when column j would make
+    the matrix indefinite (the reference throws there; the copy then holds
+    columns 0..j-1 updated, as the reference's matrix does)."""
+    lib = lib or load()
+    a = _prep(L, row_major)
+    n = a.shape[0]
+    vv = np.ascontiguousarray(np.asarray(v, dtype=np.float64))
+    assert vv.size >= (n - 1) * incv + 1 or n == 0
+    info = lib.orc_dchol_update(_ptr(a), n, max(_ld(a, row_major), 1) if n else 1, int(row_major), float(alpha),
+                                float(beta), _ptr(vv), incv)
+    return a, int(info)
 
 
 def vector_length(lib=None) -> int:

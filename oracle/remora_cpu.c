@@ -16,6 +16,7 @@
  *   ugemm / mgemm / pack_A / pack_B .... include/remora/kernels/default/mgemm.hpp:50-187
  *   vector width (32 B with AVX) ....... include/remora/kernels/default/simd.hpp:49-72
  *   cholesky_decomposition::solve ...... include/remora/decompositions.hpp:101-115
+ *   cholesky_decomposition::update ..... include/remora/decompositions.hpp:132-169
  *
  * Matrices are strided views: element (i,j) lives at p[i*s1 + j*s2] and the
  * view carries Remora's orientation tag (row_major: s2 is the unit stride).
@@ -490,6 +491,58 @@ int orc_dposv(double *A, size_t n, size_t lda, int a_row_major, double *B, size_
         if (!g_trsm_singular) kernels_trsm(L, Bv, 0, 0, 0);
     }
     return g_trsm_singular;
+}
+
+/* cholesky_decomposition::update(alpha, beta, v) (decompositions.hpp:132-169):
+ * the lower factor L of A becomes that of alpha*A + beta*v*v^T, column by
+ * column, in the reference's order of operations.  beta == 0 scales the
+ * whole stored matrix by sqrt(alpha), as the reference does.  Returns 0, or
+ * j+1 when column j would make the matrix indefinite (the reference throws
+ * std::invalid_argument there, after updating columns 0..j-1). */
+size_t orc_dchol_update(double *Lp, size_t n, size_t ldl, int row_major, double alpha, double beta,
+                        const double *v, size_t incv) {
+    view L = make_view(Lp, n, n, ldl, row_major);
+    if (beta == 0) {
+        const double s = sqrt(alpha);
+        for (size_t i = 0; i < n; ++i)
+            for (size_t j = 0; j < n; ++j) *at(L, i, j) *= s;
+        return 0;
+    }
+    double *temp = (double *)malloc((n ? n : 1) * sizeof(double));
+    for (size_t i = 0; i < n; ++i) temp[i] = v[i * incv];
+    double beta_prime = 1;
+    const double a = sqrt(alpha);
+    size_t info = 0;
+    for (size_t j = 0; j != n; ++j) {
+        const double Ljj = a * *at(L, j, j);
+        const double dj = Ljj * Ljj;
+        const double wj = temp[j];
+        const double swj2 = beta * wj * wj;
+        const double gamma = dj * beta_prime + swj2;
+        const double x = dj + swj2 / beta_prime;
+        if (x <= 0.0) {
+            info = j + 1;
+            break;
+        }
+        const double nLjj = sqrt(x);
+        *at(L, j, j) = nLjj;
+        beta_prime += swj2 / dj;
+        if (j + 1 < n) {
+#ifdef ORC_MUTANT_UPDATE_UNSCALED_TEMP
+            for (size_t i = j + 1; i < n; ++i) temp[i] -= (wj / Ljj) * *at(L, i, j);
+            for (size_t i = j + 1; i < n; ++i) *at(L, i, j) *= a;
+#else
+            for (size_t i = j + 1; i < n; ++i) *at(L, i, j) *= a;
+            for (size_t i = j + 1; i < n; ++i) temp[i] -= (wj / Ljj) * *at(L, i, j);
+#endif
+            if (gamma == 0) continue;
+            for (size_t i = j + 1; i < n; ++i) *at(L, i, j) *= nLjj / Ljj;
+            const double f = nLjj * beta * wj / gamma;
+            for (size_t i = j + 1; i < n; ++i) *at(L, i, j) += f * temp[i];
+        }
+    }
+    free(temp);
+    return info;
 }
 
 /* The build's SIMD width, for tests that pin the block constants. */

@@ -559,6 +559,17 @@ rmh_status adjoint_entry(rmh_handle h, int upper, size_t n, const T* L, size_t l
                                  (int64_t)ldab, hs(stream));
 }
 template <typename T>
+rmh_status chol_update_entry(rmh_handle h, int upper, size_t n, T alpha, T beta, T* L, size_t ldl, const T* v,
+                             size_t incv, int64_t* d_info, rmh_stream_t stream) {
+    RMH_CHECK_HANDLE(h);
+    RMH_REQUIRE(ldl >= mx1(n), "chol_update: ldl too small");
+    RMH_REQUIRE(incv >= 1, "chol_update: incv must be positive");
+    RMH_REQUIRE(d_info != nullptr, "chol_update: d_info is NULL");
+    RMH_REQUIRE(n == 0 || (L != nullptr && (v != nullptr || beta == T(0))), "chol_update: NULL pointer");
+    return chol_update_impl<T>(ctx(h), upper != 0, (int64_t)n, alpha, beta, L, (int64_t)ldl, v, (int64_t)incv, d_info,
+                               hs(stream));
+}
+template <typename T>
 rmh_status potrf_entry(rmh_handle h, int upper, size_t n, T* A, size_t lda, int64_t* d_info, rmh_stream_t stream) {
     RMH_CHECK_HANDLE(h);
     RMH_REQUIRE(lda >= mx1(n), "potrf: lda too small");
@@ -695,6 +706,15 @@ rmh_status rmh_dsposv(rmh_handle h, int upper, rmh_factor_precision prec, size_t
     RMH_REQUIRE(A != nullptr && B != nullptr && X != nullptr, "dsposv: NULL pointer");
     return dsposv_impl(ctx(h), upper != 0, prec, (int64_t)n, (int64_t)nrhs, A, (int64_t)lda, B, (int64_t)ldb, X,
                        (int64_t)ldx, tol, max_iter, iters, berr, hs(stream));
+}
+
+rmh_status rmh_dchol_update(rmh_handle h, int upper, size_t n, double alpha, double beta, double* L, size_t ldl,
+                            const double* v, size_t incv, int64_t* d_info, rmh_stream_t stream) {
+    return chol_update_entry<double>(h, upper, n, alpha, beta, L, ldl, v, incv, d_info, stream);
+}
+rmh_status rmh_schol_update(rmh_handle h, int upper, size_t n, float alpha, float beta, float* L, size_t ldl,
+                            const float* v, size_t incv, int64_t* d_info, rmh_stream_t stream) {
+    return chol_update_entry<float>(h, upper, n, alpha, beta, L, ldl, v, incv, d_info, stream);
 }
 
 // ----------------------------------------------------- differentiation --

@@ -19,7 +19,7 @@ struct ProfRec {
 
 // Scratch slots of a handle: each grows on demand and is reused by every
 // call that uses it.  Routines that call other routines use different slots.
-enum WsSlot { WS_MAIN = 0, WS_MIXED = 1, WS_FACTOR = 2, WS_ADJOINT = 3, WS_COUNT = 4 };
+enum WsSlot { WS_MAIN = 0, WS_MIXED = 1, WS_FACTOR = 2, WS_ADJOINT = 3, WS_UPDATE = 4, WS_COUNT = 5 };
 
 struct Context {
     // default panel width by precision: FP64 trailing updates gain from
@@ -30,8 +30,8 @@ struct Context {
     int device = 0;
     void* ws = nullptr;  // WS_MAIN (potrf / trsm scratch)
     size_t ws_bytes = 0;
-    void* aux[WS_COUNT] = {nullptr, nullptr, nullptr, nullptr};
-    size_t aux_bytes[WS_COUNT] = {0, 0, 0, 0};
+    void* aux[WS_COUNT] = {};
+    size_t aux_bytes[WS_COUNT] = {};
     size_t nb = 0, nbi = kDefaultNbi;  // nb 0: the default for the precision
     int prof_mask = 0;  // bit i: record launches of ProfClass i
     bool lookahead = true;
@@ -116,5 +116,9 @@ rmh_status batched_autotune_impl(Context* h, int64_t batch, const int64_t* h_n, 
 template <typename T>
 rmh_status potrf_adjoint_impl(Context* h, bool upper, int64_t n, const T* L, int64_t ldl, const T* Lbar, int64_t ldlb,
                               T* Abar, int64_t ldab, hipStream_t s);
+
+template <typename T>
+rmh_status chol_update_impl(Context* h, bool upper, int64_t n, T alpha, T beta, T* L, int64_t ldl, const T* v,
+                            int64_t incv, int64_t* info, hipStream_t s);
 
 }  // namespace rmh

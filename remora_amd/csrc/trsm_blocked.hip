@@ -121,6 +121,89 @@ int update_splits(int64_t M, int64_t N, int64_t K) {
 
 constexpr int kMaxInvSplit = 64, kMaxUpdSplit = 8;
 
+// One right-hand side: y[0:M] -= op(A) x with op(A) M x K, A column-major
+// (TR == false) or its transpose (TR == true: op(A)(i,k) = A[k + i*lda]).  A
+// workgroup takes 256 rows and one depth chunk, a thread one row; the
+// transposed operand is staged through LDS 16 depths at a time so that 16
+// consecutive lanes read 16 consecutive elements.  With several chunks the
+// partial sums go to P and an ordered reduction subtracts them.  The MFMA
+// GEMM would spend 63 of every 64 columns of its tiles on padding here.
+constexpr int kGvRows = 256, kGvStage = 16;
+
+template <typename T, bool TR>
+__global__ __launch_bounds__(kGvRows) void gemv_update_kernel(int64_t M, int64_t K, int64_t kc,
+                                                              const T* __restrict__ A, int64_t lda,
+                                                              const T* __restrict__ x, T* __restrict__ y,
+                                                              T* __restrict__ P) {
+    __shared__ T stage[TR ? kGvStage : 1][kGvRows + 1];
+    const int tid = threadIdx.x;
+    const int64_t i0 = (int64_t)blockIdx.x * kGvRows, i = i0 + tid;
+    const int64_t k0 = (int64_t)blockIdx.y * kc, k1 = min(K, k0 + kc);
+    const bool ok = i < M;
+    T acc = T(0);
+    for (int64_t kk = k0; kk < k1; kk += kGvStage) {
+        const int kw = (int)min((int64_t)kGvStage, k1 - kk);
+        T a[kGvStage];
+        if constexpr (!TR) {
+#pragma unroll
+            for (int s = 0; s < kGvStage; ++s) a[s] = (ok && s < kw) ? A[i + (kk + s) * lda] : T(0);
+        } else {
+            __syncthreads();
+            const int kq = tid % kGvStage, r0 = tid / kGvStage;
+#pragma unroll
+            for (int s = 0; s < kGvRows / (kGvRows / kGvStage); ++s) {
+                const int rr = r0 + s * (kGvRows / kGvStage);
+                stage[kq][rr] = (i0 + rr < M && kq < kw) ? A[(kk + kq) + (i0 + rr) * lda] : T(0);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < kGvStage; ++s) a[s] = stage[s][tid];
+        }
+#pragma unroll
+        for (int s = 0; s < kGvStage; ++s)
+            if (s < kw) acc += a[s] * x[min(kk + s, K - 1)];
+    }
+    if (!ok) return;
+    if (gridDim.y == 1) y[i] -= acc;
+    else P[(int64_t)blockIdx.y * M + i] = acc;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void gemv_update_reduce_kernel(int64_t M, int splits, const T* __restrict__ P,
+                                                                 T* __restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= M) return;
+    T s = T(0);
+    for (int sp = 0; sp < splits; ++sp) s += P[(int64_t)sp * M + i];
+    y[i] -= s;
+}
+
+// y -= op(A) x (see gemv_update_kernel); P holds p_el elements
+template <typename T>
+rmh_status launch_gemv_update(int64_t M, int64_t K, const T* A, int64_t lda, bool trans, const T* x, T* y, T* P,
+                              size_t p_el, hipStream_t s) {
+    if (M <= 0 || K <= 0) return RMH_STATUS_SUCCESS;
+    const int64_t row_tiles = (M + kGvRows - 1) / kGvRows;
+    // about four workgroups per compute unit, within the partial workspace
+    int64_t splits = (1024 + row_tiles - 1) / row_tiles;
+    splits = std::min<int64_t>(splits, std::max<int64_t>(1, K / 64));
+    splits = std::min<int64_t>(splits, std::max<int64_t>(1, (int64_t)(p_el / (size_t)M)));
+    int64_t kc = (K + splits - 1) / splits;
+    kc = (kc + kGvStage - 1) / kGvStage * kGvStage;
+    splits = (K + kc - 1) / kc;
+    if (splits > 1 && (size_t)splits * (size_t)M > p_el) return fail(RMH_STATUS_INVALID_VALUE, "trsv: split workspace");
+    const dim3 grid((unsigned)row_tiles, (unsigned)splits);
+    if (trans)
+        hipLaunchKernelGGL((gemv_update_kernel<T, true>), grid, dim3(kGvRows), 0, s, M, K, kc, A, lda, x, y, P);
+    else
+        hipLaunchKernelGGL((gemv_update_kernel<T, false>), grid, dim3(kGvRows), 0, s, M, K, kc, A, lda, x, y, P);
+    RMH_TRY(check_launch("gemv_update_kernel"));
+    if (splits == 1) return RMH_STATUS_SUCCESS;
+    hipLaunchKernelGGL(gemv_update_reduce_kernel<T>, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, M,
+                       (int)splits, P, y);
+    return check_launch("gemv_update_reduce_kernel");
+}
+
 template <typename T>
 size_t partial_elems(int64_t na, int64_t other, int64_t nbd) {
     return std::max((size_t)kMaxInvSplit * nbd * other, (size_t)kMaxUpdSplit * na * other);
@@ -215,6 +298,13 @@ rmh_status trsm_apply(const TrsmInverses<T>& inv, bool left, bool trans, int64_t
         if (u1 <= u0) continue;
         GemmDesc<T> u;
         u.alpha = T(-1); u.beta = T(1);
+        if (left && other == 1) {  // one right-hand side: a split GEMV
+            const T* ap = nullptr;
+            bool tr = false;
+            opA(u0, r0, &ap, &tr);
+            RMH_TRY(launch_gemv_update<T>(u1 - u0, kb, ap, lda, tr, B + r0, B + u0, P, p_el, s));
+            continue;
+        }
         if (left) {  // B[u0:u1] -= op(A)[u0:u1, r0:r0+kb] X_b
             u.M = u1 - u0; u.N = other; u.K = kb;
             opA(u0, r0, &u.A, &u.transA);

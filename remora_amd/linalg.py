@@ -126,6 +126,20 @@ def dsposv(h, A, B, X, upper=False, prec="fp32", tol=1e-14, max_iter=30, stream=
     return it.value, be.value
 
 
+def chol_update(h, L, alpha, beta, v, upper=False, info=None, stream=None):
+    """In-place rank-one update of a Cholesky factor: L L^T <- alpha L L^T +
+    beta v v^T (v: a 1-D device tensor, any stride).  Returns the device int64
+    info tensor (0, or j+1 for the column that would make it indefinite)."""
+    n = L.shape[0]
+    if info is None:
+        info = torch.zeros(1, dtype=torch.int64, device=L.device)
+    f = _fn("rmh_", L.dtype, "chol_update")
+    inc = v.stride(0) if v.dim() == 1 and v.numel() > 0 else 1
+    capi.check(f(h.value, int(upper), n, float(alpha), float(beta), _p(L), _ld(L), _p(v), inc, _p(info),
+                 _stream(stream)))
+    return info
+
+
 def potrf_adjoint(h, L, Lbar, Abar=None, upper=False, stream=None):
     """Abar = sym(L^-T Phi(L^T Lbar) L^-1): gradient w.r.t. A of a scalar
     whose gradient w.r.t. the Cholesky factor is Lbar."""

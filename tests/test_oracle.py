@@ -238,6 +238,88 @@ def test_block_constants():
     assert orc.simd_enabled()
 
 
+# --------------------------------------------------------------- update --
+def _update_case(n, alpha, beta, seed=0):
+    rng = np.random.default_rng(seed)
+    G = rng.standard_normal((n, n))
+    A = G @ G.T / n + 0.5 * np.eye(n)
+    u = rng.standard_normal(n)
+    return A, 2.0 * u / np.linalg.norm(u)
+
+
+@pytest.mark.parametrize("row_major", ORIENT)
+def test_update_reference_fixture(row_major):
+    """Test/cholesky.cpp:146-167: n = 123, v_i = 0.1/123*i - 0.5, alpha 0.1,
+    beta 0.5; the updated factor equals the factor of alpha A + beta v v^T to
+    1e-11 percent."""
+    n = 123
+    A = fixtures.create_symm(n)
+    v = 0.1 / n * np.arange(n) - 0.5
+    F, info = orc.potrf(A, row_major=row_major)
+    assert info == 0
+    U, uinfo = orc.chol_update(F, 0.1, 0.5, v, row_major=row_major)
+    R, rinfo = orc.potrf(0.1 * A + 0.5 * np.outer(v, v), row_major=row_major)
+    assert uinfo == 0 and rinfo == 0
+    lo = np.tril_indices(n)
+    assert np.allclose(U[lo], R[lo], rtol=1e-13, atol=0)
+    # the strict upper triangle (A's values in the reference's storage) is untouched
+    up = np.triu_indices(n, 1)
+    assert np.array_equal(U[up], F[up])
+
+
+@pytest.mark.parametrize("n", [1, 2, 17, 64, 200])
+@pytest.mark.parametrize("alpha,beta", [(1.0, 1.0), (2.5, 0.3), (0.8, -0.05), (0.1, 7.0)])
+def test_update_matches_refactorisation(n, alpha, beta):
+    """Closed form: the result is THE Cholesky factor of alpha A + beta v v^T
+    (unique), so it equals LAPACK's and reproduces the updated matrix."""
+    A, v = _update_case(n, alpha, beta)
+    L = np.linalg.cholesky(A)
+    U, info = orc.chol_update(L, alpha, beta, v, row_major=False)
+    M = alpha * A + beta * np.outer(v, v)
+    assert info == 0
+    Lu = np.tril(U)
+    assert np.max(np.abs(Lu - np.linalg.cholesky(M))) <= 1e-13 * np.max(np.abs(Lu))
+    assert np.linalg.norm(Lu @ Lu.T - M) <= 1e-14 * n * np.linalg.norm(M)
+
+
+def test_update_beta_zero_scales_whole_storage():
+    """beta == 0 multiplies the whole stored matrix by sqrt(alpha)
+    (decompositions.hpp:145-148)."""
+    F = np.arange(16.0).reshape(4, 4) + 1
+    U, info = orc.chol_update(F, 4.0, 0.0, np.ones(4))
+    assert info == 0 and np.array_equal(U, 2.0 * F)
+
+
+def test_update_indefinite_reports_first_failing_column():
+    """A downdate that leaves alpha A + beta v v^T indefinite: info is the
+    order of the first leading minor that is not positive definite (checked by
+    brute force), and the columns before it hold that matrix's partial
+    factor, as the reference leaves them when it throws."""
+    n = 60
+    A, v = _update_case(n, 1.0, -10.0, seed=3)
+    M = A - 10.0 * np.outer(v, v)
+    first = next(k for k in range(1, n + 1) if np.min(np.linalg.eigvalsh(M[:k, :k])) <= 0)
+    U, info = orc.chol_update(np.linalg.cholesky(A), 1.0, -10.0, v, row_major=False)
+    assert info == first
+    j = info - 1
+    Lp = np.linalg.cholesky(M[:j, :j]) if j else np.zeros((0, 0))
+    # columns 0..j-1 of the factor of M: the leading block and the rows below
+    below = np.linalg.solve(Lp, M[:j, j:]).T if j else np.zeros((n, 0))
+    assert np.allclose(np.tril(U[:j, :j]), Lp, rtol=1e-11, atol=1e-12)
+    assert np.allclose(U[j:, :j], below, rtol=1e-10, atol=1e-11)
+
+
+@pytest.mark.parametrize("incv", [1, 3])
+def test_update_strided_vector(incv):
+    A, v = _update_case(30, 1.5, 0.4, seed=5)
+    vs = np.full(30 * incv, np.nan)
+    vs[::incv] = v
+    L = np.linalg.cholesky(A)
+    U1, _ = orc.chol_update(L, 1.5, 0.4, v, row_major=False)
+    U2, _ = orc.chol_update(L, 1.5, 0.4, vs, row_major=False, incv=incv)
+    assert np.array_equal(U1, U2)
+
+
 # -------------------------------------------------------------- mutants --
 @pytest.fixture(scope="module")
 def mutant_dir(tmp_path_factory):
@@ -280,3 +362,15 @@ def test_mutant_syrk_triangle_is_caught(mutant_dir):
     M = orc.syrk(arg, np.full((384, 384), 3.0), 2.0, False, lib=lib)
     mask = np.tril(np.ones((384, 384), bool))
     assert not (np.allclose(M[mask], (2.0 * arg @ arg.T + 3.0)[mask], rtol=1e-12) and np.all(M[~mask] == 3.0))
+
+
+def test_mutant_update_unscaled_temp_is_caught(mutant_dir):
+    lib = _mutant(mutant_dir, "UPDATE_UNSCALED_TEMP")
+    n = 123
+    A = fixtures.create_symm(n)
+    v = 0.1 / n * np.arange(n) - 0.5
+    F, _ = orc.potrf(A)
+    U, info = orc.chol_update(F, 0.1, 0.5, v, lib=lib)
+    R, _ = orc.potrf(0.1 * A + 0.5 * np.outer(v, v))
+    lo = np.tril_indices(n)
+    assert info != 0 or not np.allclose(U[lo], R[lo], rtol=1e-13, atol=0)
