@@ -72,6 +72,14 @@ inline rmh_status potrf(rmh_handle h, int up, size_t n, double* A, size_t lda, i
 inline rmh_status potrf(rmh_handle h, int up, size_t n, float* A, size_t lda, int64_t* info, rmh_stream_t s) {
     return rmh_spotrf(h, up, n, A, lda, info, s);
 }
+inline rmh_status chol_update(rmh_handle h, int up, size_t n, double al, double be, double* L, size_t ldl,
+                              const double* v, size_t incv, int64_t* info, rmh_stream_t s) {
+    return rmh_dchol_update(h, up, n, al, be, L, ldl, v, incv, info, s);
+}
+inline rmh_status chol_update(rmh_handle h, int up, size_t n, float al, float be, float* L, size_t ldl,
+                              const float* v, size_t incv, int64_t* info, rmh_stream_t s) {
+    return rmh_schol_update(h, up, n, al, be, L, ldl, v, incv, info, s);
+}
 }  // namespace abi
 }  // namespace hip
 

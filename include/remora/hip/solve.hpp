@@ -14,6 +14,8 @@
 #ifndef REMORA_HIP_SOLVE_HPP
 #define REMORA_HIP_SOLVE_HPP
 
+#include <stdexcept>
+
 #include "kernels.hpp"
 
 namespace remora {
@@ -51,6 +53,31 @@ public:
     void solve(vector_expression<VecB, hip_tag>& b, system_tag<Left>) const {
         kernels::trsv<lower, left>(lower_factor(), b);
         kernels::trsv<upper, left>(upper_factor(), b);
+    }
+
+    // Rank-one update A <- alpha A + beta v v^T of the factorised matrix, in
+    // O(n^2) on the device (decompositions.hpp:132-169).  Throws
+    // std::invalid_argument, as the reference does, when the update would
+    // make the matrix indefinite; the factor is then left unchanged.
+    template <class VecV>
+    void update(value_type alpha, value_type beta, vector_expression<VecV, hip_tag> const& v) {
+        hip::device& q = m_cholesky.queue();
+        q.set_device();
+        auto sl = m_cholesky.raw_storage();
+        auto sv = v().raw_storage();
+        // row-major storage is the transposed column-major matrix: its lower
+        // factor is stored as an upper one
+        const bool up = MatrixStorage::orientation::is_row_major;
+        hip::buffer<int64_t> info(1, q);
+        hip::stream& s = hip::get_stream(q);
+        hip::check_rmh(hip::abi::chol_update(hip::get_handle(q), up, m_cholesky.size1(), alpha, beta, sl.values,
+                                             sl.leading_dimension, sv.values, sv.stride, info.get(), s.abi()));
+        int64_t h = 0;
+        hip::check_hip(hipMemcpyAsync(&h, info.get(), sizeof(h), hipMemcpyDeviceToHost, s.handle()), "hipMemcpyAsync");
+        s.synchronize();
+        if (h != 0)
+            throw std::invalid_argument(
+                "[cholesky_decomposition::update] update makes matrix indefinite, no update available");
     }
 
 private:

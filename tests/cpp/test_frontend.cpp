@@ -248,6 +248,46 @@ void symm_solve_test() {
     CHECK_SMALL(verr, 1e-12);
 }
 
+// Test/cholesky.cpp:146-167: n = 123, v_i = 0.1/123*i - 0.5, alpha 0.1,
+// beta 0.5; dec.update(alpha, beta, v) equals the factor of the updated
+// matrix to 1e-11 percent.  Then a downdate that makes the matrix indefinite
+// throws std::invalid_argument.
+template <class O>
+static void update_test() {
+    const std::size_t n = 123;
+    auto A = create_symm<O>(n, false);
+    matrix<double, O, cpu_tag> Au(n, n);
+    std::vector<double> v(n);
+    for (std::size_t i = 0; i != n; ++i) v[i] = 0.1 / n * i - 0.5;
+    for (std::size_t i = 0; i != n; ++i)
+        for (std::size_t j = 0; j != n; ++j) Au(i, j) = 0.1 * A(i, j) + 0.5 * v[i] * v[j];
+    auto dA = copy_to_device(A, hip_tag());
+    auto dAu = copy_to_device(Au, hip_tag());
+    vector<double, hip_tag> dv(n);
+    hip::check_hip(hipMemcpy(dv.raw_storage().values, v.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    cholesky_decomposition<matrix<double, O, hip_tag>> dec(dA), decUpdate(dAu);
+    dec.update(0.1, 0.5, dv);
+    auto L = copy_to_cpu(dec.lower_factor()), R = copy_to_cpu(decUpdate.lower_factor());
+    for (std::size_t i = 0; i != n; ++i)
+        for (std::size_t j = 0; j <= i; ++j) CHECK_CLOSE_PCT(L(i, j), R(i, j), 1e-11);
+    bool threw = false;
+    try {
+        dec.update(1.0, -1.0e6, dv);
+    } catch (std::invalid_argument const&) {
+        threw = true;
+    }
+    ++g_checks;
+    if (!threw) {
+        std::printf("  FAIL %s:%d indefinite downdate did not throw\n", __FILE__, __LINE__);
+        ++g_fail;
+    }
+    auto L2 = copy_to_cpu(dec.lower_factor());
+    double same = 0;
+    for (std::size_t i = 0; i != n; ++i)
+        for (std::size_t j = 0; j != n; ++j) same = std::fmax(same, std::fabs(L2(i, j) - L(i, j)));
+    CHECK_SMALL(same, 0.0);
+}
+
 static void group(const char* name, void (*fn)()) {
     int before = g_fail, cbefore = g_checks;
     try {
@@ -261,6 +301,8 @@ static void group(const char* name, void (*fn)()) {
 
 int main() {
     std::setvbuf(stdout, nullptr, _IONBF, 0);
+    group("update row_major", update_test<row_major>);
+    group("update column_major", update_test<column_major>);
     group("potrf row_major", potrf_test<row_major>);
     group("potrf column_major", potrf_test<column_major>);
     group("syrk row_major", syrk_test<row_major>);
