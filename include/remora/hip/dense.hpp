@@ -14,6 +14,8 @@
 
 #include <algorithm>
 #include <cassert>
+#include <cstring>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -235,6 +237,68 @@ private:
 // Host <-> device copies with the layout preserved (one contiguous copy on
 // the thread's stream, then a synchronisation; the reference used a
 // zero-copy kernel over registered host memory, hip/copy.hpp:362-386).
+//
+// Downloads of 16 MB and more go through two pinned 64 MB staging buffers:
+// the DMA of chunk k+1 runs while the CPU copies chunk k out, which measured
+// 27 GB/s against 17 GB/s for a copy into pageable memory on MI355X (uploads
+// measured the same either way, 27-28 GB/s, and stay direct).  The buffers
+// are allocated on first use and kept for the life of the process; a mutex
+// serialises their use across threads.
+namespace hip {
+namespace detail {
+
+struct download_staging {
+    static constexpr std::size_t chunk = std::size_t(64) << 20;
+    std::mutex lock;
+    void* pinned[2] = {nullptr, nullptr};
+    static download_staging& instance() {
+        static download_staging* st = new download_staging();  // kept until the process ends
+        return *st;
+    }
+};
+
+inline void download(void* dst, const void* src, std::size_t bytes, stream& s) {
+    if (bytes < (std::size_t(16) << 20)) {
+        check_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s.handle()), "hipMemcpyAsync");
+        s.synchronize();
+        return;
+    }
+    download_staging& st = download_staging::instance();
+    std::lock_guard<std::mutex> guard(st.lock);
+    for (int b = 0; b < 2; ++b)
+        if (st.pinned[b] == nullptr)
+            check_hip(hipHostMalloc(&st.pinned[b], download_staging::chunk, hipHostMallocDefault), "hipHostMalloc");
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    check_hip(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming), "hipEventCreateWithFlags");
+    check_hip(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming), "hipEventCreateWithFlags");
+    const std::size_t nch = (bytes + download_staging::chunk - 1) / download_staging::chunk;
+    auto len = [&](std::size_t k) { return std::min(download_staging::chunk, bytes - k * download_staging::chunk); };
+    auto issue = [&](std::size_t k) {
+        check_hip(hipMemcpyAsync(st.pinned[k & 1], static_cast<const char*>(src) + k * download_staging::chunk,
+                                 len(k), hipMemcpyDeviceToHost, s.handle()),
+                  "hipMemcpyAsync");
+        check_hip(hipEventRecord(ev[k & 1], s.handle()), "hipEventRecord");
+    };
+    try {
+        issue(0);
+        for (std::size_t k = 0; k < nch; ++k) {
+            if (k + 1 < nch) issue(k + 1);
+            check_hip(hipEventSynchronize(ev[k & 1]), "hipEventSynchronize");
+            std::memcpy(static_cast<char*>(dst) + k * download_staging::chunk, st.pinned[k & 1], len(k));
+        }
+    } catch (...) {
+        (void)hipStreamSynchronize(s.handle());
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+        throw;
+    }
+    check_hip(hipEventDestroy(ev[0]), "hipEventDestroy");
+    check_hip(hipEventDestroy(ev[1]), "hipEventDestroy");
+}
+
+}  // namespace detail
+}  // namespace hip
+
 template <class T, class O>
 matrix<T, O, hip_tag> copy_to_device(matrix<T, O, cpu_tag> const& m, hip_tag,
                                      hip::device& q = hip::device_manager::instance().default_device()) {
@@ -254,9 +318,7 @@ matrix<T, O, cpu_tag> copy_to_cpu(matrix<T, O, hip_tag> const& d) {
     if (m.size1() * m.size2() == 0) return m;
     hip::stream& s = hip::get_stream(d.queue());
     d.queue().set_device();
-    hip::check_hip(hipMemcpyAsync(m.data(), d.raw_storage().values, m.size1() * m.size2() * sizeof(T),
-                                  hipMemcpyDeviceToHost, s.handle()), "hipMemcpyAsync");
-    s.synchronize();
+    hip::detail::download(m.data(), d.raw_storage().values, m.size1() * m.size2() * sizeof(T), s);
     return m;
 }
 

@@ -288,6 +288,24 @@ static void update_test() {
     CHECK_SMALL(same, 0.0);
 }
 
+// copy_to_device / copy_to_cpu round trip exact, above the staging threshold
+// with a ragged last chunk (3000 x 3001 doubles = 72 MB: two 64 MB chunks),
+// and below it.
+template <class O>
+static void transport_test() {
+    for (std::size_t n : {3000, 300}) {
+        matrix<double, O, cpu_tag> A(n, n + 1);
+        for (std::size_t i = 0; i != n; ++i)
+            for (std::size_t j = 0; j != n + 1; ++j) A(i, j) = 0.25 * i - 1.0 / (j + 1);
+        auto dA = copy_to_device(A, hip_tag());
+        auto B = copy_to_cpu(dA);
+        double diff = 0;
+        for (std::size_t i = 0; i != n; ++i)
+            for (std::size_t j = 0; j != n + 1; ++j) diff = std::fmax(diff, std::fabs(B(i, j) - A(i, j)));
+        CHECK_SMALL(diff, 0.0);
+    }
+}
+
 static void group(const char* name, void (*fn)()) {
     int before = g_fail, cbefore = g_checks;
     try {
@@ -301,6 +319,8 @@ static void group(const char* name, void (*fn)()) {
 
 int main() {
     std::setvbuf(stdout, nullptr, _IONBF, 0);
+    group("transport row_major", transport_test<row_major>);
+    group("transport column_major", transport_test<column_major>);
     group("update row_major", update_test<row_major>);
     group("update column_major", update_test<column_major>);
     group("potrf row_major", potrf_test<row_major>);
