@@ -116,6 +116,22 @@ rmh_status rmh_dpotrs(rmh_handle h, int upper, size_t n, size_t nrhs, const doub
 rmh_status rmh_spotrs(rmh_handle h, int upper, size_t n, size_t nrhs, const float* A, size_t lda, float* B,
                       size_t ldb, rmh_stream_t stream);
 
+/* Pivoted Cholesky factorisation of a symmetric positive semi-definite
+ * matrix in place: P^T A P = L L^T (lower) or U^T U (upper), the reference's
+ * algorithm and pivot order (kernels::pstrf, include/remora/kernels/pstrf.hpp:58-66,
+ * include/remora/kernels/default/pstrf.hpp:40-160).  d_piv (device, n int64)
+ * receives the transpositions: at step j row/column j was swapped with
+ * d_piv[j] (j itself when not swapped; 0-based), as the reference's
+ * permutation_matrix records them.  d_rank (device int64) receives the rank:
+ * the factorisation stops when the largest remaining pivot is below
+ * n^2 * epsilon * max|A_ii|, and the trailing square is then zero.  Both
+ * triangles of the stored matrix are read and written, as in the reference
+ * (the part beyond the factor's triangle is zeroed).  Asynchronous. */
+rmh_status rmh_dpstrf(rmh_handle h, int upper, size_t n, double* A, size_t lda, int64_t* d_piv, int64_t* d_rank,
+                      rmh_stream_t stream);
+rmh_status rmh_spstrf(rmh_handle h, int upper, size_t n, float* A, size_t lda, int64_t* d_piv, int64_t* d_rank,
+                      rmh_stream_t stream);
+
 /* Rank-one update of a Cholesky factor in place: with A = L L^T on entry
  * (the factor from rmh_?potrf: L lower, or U = L^T upper; only that triangle
  * read and written), on return L L^T = alpha*A + beta*v*v^T.  Replaces

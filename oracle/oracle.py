@@ -49,6 +49,8 @@ def _declare(lib):
     lib.orc_dposv.restype = I
     lib.orc_dchol_update.argtypes = [P, S, S, I, D, D, P, S]
     lib.orc_dchol_update.restype = S
+    lib.orc_dpstrf.argtypes = [P, S, S, I, I, P]
+    lib.orc_dpstrf.restype = S
     lib.orc_vector_length.restype = I
     lib.orc_simd_enabled.restype = I
 
@@ -152,6 +154,32 @@ def chol_update(L, alpha: float, beta: float, v, row_major: bool = True, incv: i
     info = lib.orc_dchol_update(_ptr(a), n, max(_ld(a, row_major), 1) if n else 1, int(row_major), float(alpha),
                                 float(beta), _ptr(vv), incv)
     return a, int(info)
+
+
+def pstrf(A, upper: bool = False, row_major: bool = True, lib=None):
+    """kernels::pstrf<lower|upper>(A, P) (kernels/default/pstrf.hpp:40-160).
+
+    Returns (factored copy of A, P, rank): P starts as the identity and holds
+    the row swapped with j at step j (the reference's permutation_matrix)."""
+    lib = lib or load()
+    a = _prep(A, row_major)
+    n = a.shape[0]
+    P = np.arange(n, dtype=np.int64)
+    rank = lib.orc_dpstrf(_ptr(a), n, max(_ld(a, row_major), 1) if n else 1, int(row_major), int(upper),
+                          _ptr(P) if n else None)
+    return a, P, int(rank)
+
+
+def swap_full_inverted(P, M):
+    """P^T M P for a transposition record P (the reference's
+    swap_full_inverted: swaps applied in reverse order to rows and columns)."""
+    M = np.array(M, copy=True)
+    for i in range(len(P) - 1, -1, -1):
+        p = int(P[i])
+        if p != i:
+            M[[i, p], :] = M[[p, i], :]
+            M[:, [i, p]] = M[:, [p, i]]
+    return M
 
 
 def vector_length(lib=None) -> int:

@@ -17,6 +17,7 @@
  *   vector width (32 B with AVX) ....... include/remora/kernels/default/simd.hpp:49-72
  *   cholesky_decomposition::solve ...... include/remora/decompositions.hpp:101-115
  *   cholesky_decomposition::update ..... include/remora/decompositions.hpp:132-169
+ *   pstrf (pivoted, block 20) .......... include/remora/kernels/default/pstrf.hpp:40-160
  *
  * Matrices are strided views: element (i,j) lives at p[i*s1 + j*s2] and the
  * view carries Remora's orientation tag (row_major: s2 is the unit stride).
@@ -543,6 +544,86 @@ size_t orc_dchol_update(double *Lp, size_t n, size_t ldl, int row_major, double 
     }
     free(temp);
     return info;
+}
+
+/* kernels::pstrf<lower>(A, P) (kernels/default/pstrf.hpp:40-148; upper is
+ * the same on trans(A), :150-158): pivoted Cholesky in blocks of 20 columns.
+ * Per column the pivot is the first largest running diagonal value; rows and
+ * then columns of the whole matrix are swapped; the column takes the block's
+ * deferred updates (a GEMV) and is scaled; the row right of the diagonal is
+ * cleared.  After each block the whole trailing square takes -= L21 L21^T.
+ * Stops when the pivot drops below n^2 * eps * max|A_ii|, clearing the
+ * trailing square.  P (identity on entry, as permutation_matrix(n) is) gets
+ * P[j] = pivot row for every swap.  Returns the rank. */
+size_t orc_dpstrf(double *Ap, size_t m, size_t lda, int row_major, int upper, int64_t *P) {
+    view A = make_view(Ap, m, m, lda, row_major);
+    if (upper) A = trans(A);
+    const size_t block_size = 20;
+    double *pv = (double *)malloc((m ? m : 1) * sizeof(double));
+    double max_diag = m ? *at(A, 0, 0) : 0.0;
+    for (size_t i = 1; i < m; ++i) max_diag = fmax(max_diag, fabs(*at(A, i, i)));
+    const double epsilon = (double)m * (double)m * 2.220446049250313e-16 * max_diag;
+    for (size_t k = 0; k < m; k += block_size) {
+        const size_t cur = min_sz(m - k, block_size);
+        view Ak = subrange(A, k, m, k, m);
+        double *piv = pv + k;
+        const size_t mk = m - k;
+        for (size_t j = 0; j != cur; ++j) {
+            if (j == 0) {
+                for (size_t i = 0; i != mk; ++i) piv[i] = *at(Ak, i, i);
+            } else {
+                for (size_t i = j; i != mk; ++i) piv[i] -= *at(Ak, i, j - 1) * *at(Ak, i, j - 1);
+            }
+            size_t pivot = j;
+#ifdef ORC_MUTANT_PSTRF_LAST_MAX
+            for (size_t i = j + 1; i < mk; ++i)
+                if (piv[i] >= piv[pivot]) pivot = i;
+#else
+            for (size_t i = j + 1; i < mk; ++i)
+                if (piv[pivot] < piv[i]) pivot = i;
+#endif
+            if (pivot != j) {
+                P[k + j] = (int64_t)(pivot + k);
+                for (size_t t = 0; t < m; ++t) {
+                    double x = *at(A, k + j, t);
+                    *at(A, k + j, t) = *at(A, k + pivot, t);
+                    *at(A, k + pivot, t) = x;
+                }
+                for (size_t t = 0; t < m; ++t) {
+                    double x = *at(A, t, k + j);
+                    *at(A, t, k + j) = *at(A, t, k + pivot);
+                    *at(A, t, k + pivot) = x;
+                }
+                double x = piv[j];
+                piv[j] = piv[pivot];
+                piv[pivot] = x;
+            }
+            const double pivotValue = piv[j];
+            if (pivotValue < epsilon) {
+                for (size_t r = j; r < mk; ++r)
+                    for (size_t c = j; c < mk; ++c) *at(Ak, r, c) = 0.0;
+                free(pv);
+                return k + j;
+            }
+            *at(Ak, j, j) = sqrt(pivotValue);
+            if (j > 0) {
+                for (size_t r = j + 1; r < mk; ++r) {
+                    double acc = 0.0;
+                    for (size_t t = 0; t < j; ++t) acc += *at(Ak, r, t) * *at(Ak, j, t);
+                    *at(Ak, r, j) -= acc;
+                }
+            }
+            for (size_t r = j + 1; r < mk; ++r) *at(Ak, r, j) /= *at(Ak, j, j);
+            for (size_t c = j + 1; c < mk; ++c) *at(Ak, j, c) = 0.0;
+        }
+        if (k + cur < m) {
+            view LL = subrange(Ak, block_size, mk, 0, block_size);
+            view LR = subrange(Ak, block_size, mk, block_size, mk);
+            kernels_gemm(LL, trans(LL), LR, -1.0);
+        }
+    }
+    free(pv);
+    return m;
 }
 
 /* The build's SIMD width, for tests that pin the block constants. */

@@ -559,6 +559,16 @@ rmh_status adjoint_entry(rmh_handle h, int upper, size_t n, const T* L, size_t l
                                  (int64_t)ldab, hs(stream));
 }
 template <typename T>
+rmh_status pstrf_entry(rmh_handle h, int upper, size_t n, T* A, size_t lda, int64_t* d_piv, int64_t* d_rank,
+                       rmh_stream_t stream) {
+    RMH_CHECK_HANDLE(h);
+    RMH_REQUIRE(lda >= mx1(n), "pstrf: lda too small");
+    RMH_REQUIRE(lda <= (size_t(1) << 23), "pstrf: leading dimension above 2^23");
+    RMH_REQUIRE(d_rank != nullptr, "pstrf: d_rank is NULL");
+    RMH_REQUIRE(n == 0 || (A != nullptr && d_piv != nullptr), "pstrf: NULL pointer");
+    return pstrf_impl<T>(ctx(h), upper != 0, (int64_t)n, A, (int64_t)lda, d_piv, d_rank, hs(stream));
+}
+template <typename T>
 rmh_status chol_update_entry(rmh_handle h, int upper, size_t n, T alpha, T beta, T* L, size_t ldl, const T* v,
                              size_t incv, int64_t* d_info, rmh_stream_t stream) {
     RMH_CHECK_HANDLE(h);
@@ -708,6 +718,14 @@ rmh_status rmh_dsposv(rmh_handle h, int upper, rmh_factor_precision prec, size_t
                        (int64_t)ldx, tol, max_iter, iters, berr, hs(stream));
 }
 
+rmh_status rmh_dpstrf(rmh_handle h, int upper, size_t n, double* A, size_t lda, int64_t* d_piv, int64_t* d_rank,
+                      rmh_stream_t stream) {
+    return pstrf_entry<double>(h, upper, n, A, lda, d_piv, d_rank, stream);
+}
+rmh_status rmh_spstrf(rmh_handle h, int upper, size_t n, float* A, size_t lda, int64_t* d_piv, int64_t* d_rank,
+                      rmh_stream_t stream) {
+    return pstrf_entry<float>(h, upper, n, A, lda, d_piv, d_rank, stream);
+}
 rmh_status rmh_dchol_update(rmh_handle h, int upper, size_t n, double alpha, double beta, double* L, size_t ldl,
                             const double* v, size_t incv, int64_t* d_info, rmh_stream_t stream) {
     return chol_update_entry<double>(h, upper, n, alpha, beta, L, ldl, v, incv, d_info, stream);

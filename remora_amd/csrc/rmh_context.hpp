@@ -19,7 +19,7 @@ struct ProfRec {
 
 // Scratch slots of a handle: each grows on demand and is reused by every
 // call that uses it.  Routines that call other routines use different slots.
-enum WsSlot { WS_MAIN = 0, WS_MIXED = 1, WS_FACTOR = 2, WS_ADJOINT = 3, WS_UPDATE = 4, WS_COUNT = 5 };
+enum WsSlot { WS_MAIN = 0, WS_MIXED = 1, WS_FACTOR = 2, WS_ADJOINT = 3, WS_UPDATE = 4, WS_PSTRF = 5, WS_COUNT = 6 };
 
 struct Context {
     // default panel width by precision: FP64 trailing updates gain from
@@ -116,6 +116,9 @@ rmh_status batched_autotune_impl(Context* h, int64_t batch, const int64_t* h_n, 
 template <typename T>
 rmh_status potrf_adjoint_impl(Context* h, bool upper, int64_t n, const T* L, int64_t ldl, const T* Lbar, int64_t ldlb,
                               T* Abar, int64_t ldab, hipStream_t s);
+
+template <typename T>
+rmh_status pstrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int64_t* piv, int64_t* rank, hipStream_t s);
 
 template <typename T>
 rmh_status chol_update_impl(Context* h, bool upper, int64_t n, T alpha, T beta, T* L, int64_t ldl, const T* v,

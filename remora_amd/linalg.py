@@ -126,6 +126,17 @@ def dsposv(h, A, B, X, upper=False, prec="fp32", tol=1e-14, max_iter=30, stream=
     return it.value, be.value
 
 
+def pstrf(h, A, upper=False, stream=None):
+    """In-place pivoted Cholesky; returns (piv, rank) as device int64 tensors
+    (piv[j]: the row swapped with j at step j)."""
+    n = A.shape[0]
+    piv = torch.empty(max(n, 1), dtype=torch.int64, device=A.device)
+    rank = torch.empty(1, dtype=torch.int64, device=A.device)
+    f = _fn("rmh_", A.dtype, "pstrf")
+    capi.check(f(h.value, int(upper), n, _p(A), _ld(A), _p(piv), _p(rank), _stream(stream)))
+    return piv[:n], rank
+
+
 def chol_update(h, L, alpha, beta, v, upper=False, info=None, stream=None):
     """In-place rank-one update of a Cholesky factor: L L^T <- alpha L L^T +
     beta v v^T (v: a 1-D device tensor, any stride).  Returns the device int64

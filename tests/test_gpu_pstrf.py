@@ -1,0 +1,110 @@
+"""Pivoted Cholesky on the GPU (rmh_?pstrf) through the C-ABI.
+
+Compared with the oracle's restatement of kernels::pstrf
+(kernels/default/pstrf.hpp:40-160) on the same inputs: the same rank and
+the same transpositions (FP64), the factor to 1e-12 of its largest entry, and
+P^T A P reconstructed to 1e-12 (the reference's own test, Test/cholesky.cpp:
+84-143, restated: createSymm(123) of full rank and of rank 50).  FP32 checks
+the rank and the reconstruction (its pivots are FP32 values, so near-equal
+ones may order differently from the FP64 oracle).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as orc
+from remora_amd import capi
+from remora_amd import linalg as la
+from tests import fixtures
+from tests.gpu_util import dev, host
+
+pytestmark = pytest.mark.gpu
+
+_H = {}
+
+
+def handle():
+    if "h" not in _H:
+        _H["h"] = capi.Handle(0)
+    return _H["h"]
+
+
+def run(A, upper, dtype=torch.float64, ld_pad=2):
+    n = A.shape[0]
+    Ad = dev(A, dtype=dtype, ld=n + ld_pad)
+    piv, rank = la.pstrf(handle(), Ad, upper=upper)
+    torch.cuda.synchronize()
+    pad = Ad.as_strided((ld_pad, n), (1, n + ld_pad), Ad.storage_offset() + n) if n and ld_pad else None
+    return host(Ad), piv.cpu().numpy(), int(rank.item()), pad
+
+
+def factor_of(F, upper):
+    return np.triu(F).T if upper else np.tril(F)
+
+
+def spd(n, seed):
+    rng = np.random.default_rng(seed)
+    G = rng.standard_normal((n, n))
+    return G @ G.T / n + np.diag(np.linspace(0.5, 3.0, n)[rng.permutation(n)])
+
+
+def check_against_oracle(A, upper, tol=1e-12):
+    F, P, r, pad = run(A, upper)
+    Fo, Po, ro = orc.pstrf(A, upper=upper, row_major=False)
+    assert r == ro
+    assert np.array_equal(P, Po), "pivot sequence differs from the oracle"
+    L, Lo = factor_of(F, upper), factor_of(Fo, upper)
+    assert np.max(np.abs(L - Lo)) <= tol * max(np.max(np.abs(Lo)), 1e-300)
+    rec = orc.swap_full_inverted(P, L @ L.T)
+    assert np.max(np.abs(A - rec)) <= 1e-12 * max(np.max(np.abs(A)), 1.0) * max(1.0, A.shape[0] / 100)
+    # beyond the factor's triangle and the rank everything is zero, as in the reference
+    assert np.all(L[:, r:] == 0)
+    other = np.triu_indices(A.shape[0], 1)
+    assert np.all((F.T if upper else F)[other] == 0)
+    assert pad is None or bool(torch.isnan(pad).all()), "leading-dimension padding changed"
+    return r
+
+
+@pytest.mark.parametrize("upper", [False, True])
+@pytest.mark.parametrize("rank", [0, 50])
+def test_pstrf_reference_fixture(rank, upper):
+    r = check_against_oracle(fixtures.create_symm(123, rank), upper)
+    assert r == (rank or 123)
+
+
+@pytest.mark.parametrize("upper", [False, True])
+@pytest.mark.parametrize("n", [1, 7, 20, 21, 100, 513])
+def test_pstrf_spd_matches_oracle(n, upper):
+    assert check_against_oracle(spd(n, n), upper) == n
+
+
+@pytest.mark.parametrize("upper", [False, True])
+def test_pstrf_semidefinite_large(upper):
+    """n = 600 of rank 200: stops at the rank with the oracle's pivots."""
+    rng = np.random.default_rng(7)
+    G = rng.standard_normal((600, 200))
+    assert check_against_oracle(G @ G.T / 200, upper, tol=1e-10) == 200
+
+
+def test_pstrf_identity_keeps_order():
+    F, P, r, _ = run(np.eye(40), False)
+    assert r == 40 and np.array_equal(P, np.arange(40)) and np.array_equal(np.tril(F), np.eye(40))
+
+
+@pytest.mark.parametrize("upper", [False, True])
+def test_spstrf_fp32(upper):
+    n = 300
+    A = spd(n, 3)
+    F, P, r, _ = run(A, upper, dtype=torch.float32)
+    assert r == n
+    L = factor_of(F, upper)
+    rec = orc.swap_full_inverted(P, L @ L.T)
+    assert np.max(np.abs(A - rec)) <= 1e-4 * np.max(np.abs(A))
+    assert np.all((P >= np.arange(n)) & (P < n))  # step j swaps j with a later row
+
+
+def test_pstrf_empty():
+    A = torch.zeros((1, 1), dtype=torch.float64, device="cuda")[:0, :0]
+    piv, rank = la.pstrf(handle(), A)
+    torch.cuda.synchronize()
+    assert int(rank.item()) == 0

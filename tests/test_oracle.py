@@ -320,6 +320,56 @@ def test_update_strided_vector(incv):
     assert np.array_equal(U1, U2)
 
 
+# ---------------------------------------------------------------- pstrf --
+def _perm_from_swaps(P):
+    perm = np.arange(len(P))
+    for i, p in enumerate(P):
+        perm[[i, p]] = perm[[p, i]]
+    return perm
+
+
+@pytest.mark.parametrize("row_major", ORIENT)
+@pytest.mark.parametrize("upper", [False, True])
+@pytest.mark.parametrize("rank", [0, 50])
+def test_pstrf_reference_fixture(rank, upper, row_major):
+    """Test/cholesky.cpp:84-143: createSymm(123) has full rank and
+    createSymm(123, 50) rank 50; L L^T (or U^T U) with the permutation undone
+    reconstructs A to 1e-12."""
+    n = 123
+    A = fixtures.create_symm(n, rank)
+    F, P, r = orc.pstrf(A, upper=upper, row_major=row_major)
+    assert r == (rank or n)
+    L = np.triu(F).T if upper else np.tril(F)
+    assert np.max(np.abs(A - orc.swap_full_inverted(P, L @ L.T))) <= 1e-12
+    # beyond the rank the factor is zero
+    assert np.all(L[:, r:] == 0)
+
+
+@pytest.mark.parametrize("n", [1, 7, 20, 21, 64, 150])
+def test_pstrf_matches_lapack(n):
+    """LAPACK dpstrf (scipy) picks the same pivots (largest remaining
+    diagonal, first on ties) and the same factor on a matrix with well
+    separated pivots."""
+    from scipy.linalg import lapack
+    rng = np.random.default_rng(n)
+    G = rng.standard_normal((n, n))
+    A = G @ G.T / n + np.diag(np.linspace(0.5, 3.0, n)[rng.permutation(n)])
+    F, P, r = orc.pstrf(A, row_major=False)
+    c, piv, rank, info = lapack.dpstrf(A, lower=1)
+    assert r == rank == n
+    assert np.array_equal(_perm_from_swaps(P), piv - 1)
+    assert np.allclose(np.tril(F), np.tril(c), rtol=1e-12, atol=1e-12)
+
+
+def test_pstrf_ties_take_the_first_maximum():
+    """A = I: every pivot ties, std::max_element takes the first, so no row
+    moves and L = I; diag(2, 3, 3, 1) pivots on the first 3."""
+    F, P, r = orc.pstrf(np.eye(5))
+    assert r == 5 and np.array_equal(P, np.arange(5)) and np.array_equal(np.tril(F), np.eye(5))
+    F, P, r = orc.pstrf(np.diag([2.0, 3.0, 3.0, 1.0]))
+    assert P[0] == 1 and r == 4
+
+
 # -------------------------------------------------------------- mutants --
 @pytest.fixture(scope="module")
 def mutant_dir(tmp_path_factory):
@@ -374,3 +424,9 @@ def test_mutant_update_unscaled_temp_is_caught(mutant_dir):
     R, _ = orc.potrf(0.1 * A + 0.5 * np.outer(v, v))
     lo = np.tril_indices(n)
     assert info != 0 or not np.allclose(U[lo], R[lo], rtol=1e-13, atol=0)
+
+
+def test_mutant_pstrf_last_max_is_caught(mutant_dir):
+    lib = _mutant(mutant_dir, "PSTRF_LAST_MAX")
+    F, P, r = orc.pstrf(np.eye(5), lib=lib)
+    assert not np.array_equal(P, np.arange(5))
