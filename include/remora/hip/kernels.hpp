@@ -20,6 +20,8 @@
 #include <cstdint>
 #include <type_traits>
 
+#include <vector>
+
 #include "dense.hpp"
 
 namespace remora {
@@ -71,6 +73,14 @@ inline rmh_status potrf(rmh_handle h, int up, size_t n, double* A, size_t lda, i
 }
 inline rmh_status potrf(rmh_handle h, int up, size_t n, float* A, size_t lda, int64_t* info, rmh_stream_t s) {
     return rmh_spotrf(h, up, n, A, lda, info, s);
+}
+inline rmh_status pstrf(rmh_handle h, int up, size_t n, double* A, size_t lda, int64_t* piv, int64_t* rank,
+                        rmh_stream_t s) {
+    return rmh_dpstrf(h, up, n, A, lda, piv, rank, s);
+}
+inline rmh_status pstrf(rmh_handle h, int up, size_t n, float* A, size_t lda, int64_t* piv, int64_t* rank,
+                        rmh_stream_t s) {
+    return rmh_spstrf(h, up, n, A, lda, piv, rank, s);
 }
 inline rmh_status chol_update(rmh_handle h, int up, size_t n, double al, double be, double* L, size_t ldl,
                               const double* v, size_t incv, int64_t* info, rmh_stream_t s) {
@@ -215,6 +225,33 @@ std::size_t potrf(matrix_container<MatA, hip_tag>& A) {
     hip::check_hip(hipMemcpyAsync(&h, info.get(), sizeof(h), hipMemcpyDeviceToHost, s.handle()), "hipMemcpyAsync");
     s.synchronize();
     return std::size_t(h);
+}
+
+// Pivoted Cholesky on the device, kernels::pstrf<lower|upper>(A, P)
+// (kernels/pstrf.hpp:58-66): A is overwritten by the factor, P (a host
+// container of n indices with P(i) assignable, as the reference's
+// permutation_matrix) receives the transpositions, and the rank is returned.
+template <class Triangular, class MatA, class VecP>
+std::size_t pstrf(matrix_container<MatA, hip_tag>& A, VecP& P) {
+    assert(A().size1() == A().size2());
+    const std::size_t n = A().size1();
+    if (n == 0) return 0;
+    hip::device& q = A().queue();
+    q.set_device();
+    const bool up = MatA::orientation::is_row_major ? !Triangular::is_upper : Triangular::is_upper;
+    auto sa = A().raw_storage();
+    hip::buffer<int64_t> piv(n, q), rank(1, q);
+    hip::stream& s = hip::get_stream(q);
+    hip::check_rmh(hip::abi::pstrf(hip::get_handle(q), up, n, sa.values, sa.leading_dimension, piv.get(), rank.get(),
+                                   s.abi()));
+    std::vector<int64_t> hp(n);
+    int64_t r = 0;
+    hip::check_hip(hipMemcpyAsync(hp.data(), piv.get(), n * sizeof(int64_t), hipMemcpyDeviceToHost, s.handle()),
+                   "hipMemcpyAsync");
+    hip::check_hip(hipMemcpyAsync(&r, rank.get(), sizeof(r), hipMemcpyDeviceToHost, s.handle()), "hipMemcpyAsync");
+    s.synchronize();
+    for (std::size_t i = 0; i != n; ++i) P(i) = hp[i];
+    return std::size_t(r);
 }
 
 }  // namespace kernels

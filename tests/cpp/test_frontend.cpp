@@ -15,6 +15,7 @@
 #include "remora/hip/solve.hpp"
 
 extern "C" size_t orc_dpotrf(double* A, size_t n, size_t lda, int row_major, int upper);
+extern "C" size_t orc_dpstrf(double* A, size_t m, size_t lda, int row_major, int upper, int64_t* P);
 
 using namespace remora;
 
@@ -288,6 +289,37 @@ static void update_test() {
     CHECK_SMALL(same, 0.0);
 }
 
+// kernels::pstrf<lower|upper> on the device equals the oracle's restatement of
+// the reference (kernels/default/pstrf.hpp:40-160) on createSymm(123): same
+// rank and transpositions, factor to 1e-12 percent (Test/cholesky.cpp:84-111).
+struct index_vector {
+    std::vector<int64_t> v;
+    explicit index_vector(std::size_t n) : v(n) {}
+    int64_t& operator()(std::size_t i) { return v[i]; }
+};
+
+template <class O>
+static void pstrf_test() {
+    const std::size_t n = 123;
+    auto A = create_symm<O>(n, false);
+    const bool rm = O::is_row_major;
+    for (int upper = 0; upper < 2; ++upper) {
+        auto dA = copy_to_device(A, hip_tag());
+        index_vector P(n);
+        std::size_t r = upper ? kernels::pstrf<remora::upper>(dA, P) : kernels::pstrf<remora::lower>(dA, P);
+        auto F = copy_to_cpu(dA);
+        auto ref = A;
+        std::vector<int64_t> Pr(n);
+        for (std::size_t i = 0; i != n; ++i) Pr[i] = (int64_t)i;
+        std::size_t rr = orc_dpstrf(ref.data(), n, n, rm, upper, Pr.data());
+        CHECK_SMALL(double(r) - double(rr), 0.0);
+        CHECK_SMALL(double(r) - double(n), 0.0);
+        for (std::size_t i = 0; i != n; ++i) CHECK_SMALL(double(P.v[i] - Pr[i]), 0.0);
+        for (std::size_t i = 0; i != n; ++i)
+            for (std::size_t j = 0; j != n; ++j) CHECK_CLOSE_PCT(F(i, j), ref(i, j), 1e-12);
+    }
+}
+
 // copy_to_device / copy_to_cpu round trip exact, above the staging threshold
 // with a ragged last chunk (3000 x 3001 doubles = 72 MB: two 64 MB chunks),
 // and below it.
@@ -319,6 +351,8 @@ static void group(const char* name, void (*fn)()) {
 
 int main() {
     std::setvbuf(stdout, nullptr, _IONBF, 0);
+    group("pstrf row_major", pstrf_test<row_major>);
+    group("pstrf column_major", pstrf_test<column_major>);
     group("transport row_major", transport_test<row_major>);
     group("transport column_major", transport_test<column_major>);
     group("update row_major", update_test<row_major>);
