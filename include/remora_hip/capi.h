@@ -124,9 +124,9 @@ rmh_status rmh_spotrs(rmh_handle h, int upper, size_t n, size_t nrhs, const floa
  * d_piv[j] (j itself when not swapped; 0-based), as the reference's
  * permutation_matrix records them.  d_rank (device int64) receives the rank:
  * the factorisation stops when the largest remaining pivot is below
- * n^2 * epsilon * max|A_ii|, and the trailing square is then zero.  Both
- * triangles of the stored matrix are read and written, as in the reference
- * (the part beyond the factor's triangle is zeroed).  Asynchronous. */
+ * n^2 * epsilon * max|A_ii|, and the trailing square is then zero.  Only
+ * the `upper`/lower triangle is read (A symmetric); the other triangle is
+ * zeroed on return, as the reference leaves it.  Asynchronous. */
 rmh_status rmh_dpstrf(rmh_handle h, int upper, size_t n, double* A, size_t lda, int64_t* d_piv, int64_t* d_rank,
                       rmh_stream_t stream);
 rmh_status rmh_spstrf(rmh_handle h, int upper, size_t n, float* A, size_t lda, int64_t* d_piv, int64_t* d_rank,

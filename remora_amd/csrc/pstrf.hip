@@ -9,10 +9,18 @@
 // and the trailing square is cleared.
 //
 // On the device: the 20 columns of a block are one workgroup's sequential
-// loop (each column is O(n) work: pivot update, an argmax reduction, two
-// swaps, a 20-term GEMV per row), the trailing update is the MFMA GEMM with
+// loop (each column is O(n) work: pivot update, an argmax reduction, the
+// swap, a 20-term GEMV per row), the trailing update is the MFMA GEMM with
 // K = 20, and a device flag set at termination makes every later launch
-// return at once.  L(i,j) of the lower view is A[i + j*lda] (lower) or
+// return at once.  Only the factor's triangle is read and updated: the
+// symmetric row-and-column swap is applied to its pieces (rows c and p left
+// of c, the two diagonal entries, column c against row p between them, the
+// columns below p), which gives the same triangle as the reference's full
+// swaps of a symmetric matrix while touching a fraction of the strided
+// elements; the reference's row clears and the other triangle are replaced
+// by one final clear of that triangle.  The row swaps of the columns left of
+// the block (finished columns, not read while the block runs) are applied
+// after the block by a parallel kernel, in the same order.  L(i,j) of the lower view is A[i + j*lda] (lower) or
 // A[j + i*lda] (upper: the factor U = L^T, as the reference runs the lower
 // code on trans(A)).
 #include <cfloat>
@@ -64,76 +72,104 @@ __global__ __launch_bounds__(kPsThreads) void pstrf_init_kernel(PsView<T> L, int
     }
 }
 
-// the columns k .. k+19 (fewer at the end) of the factorisation
+// (value, index) maximum with ties to the smaller index: the element
+// std::max_element returns (index < 0: no candidate)
 template <typename T>
-__global__ __launch_bounds__(kPsThreads) void pstrf_block_kernel(PsView<T> L, int64_t n, int64_t k, T* __restrict__ piv,
+__device__ __forceinline__ void argmax_combine(T& v, int& i, T ov, int oi) {
+    if (oi < 0) return;
+    if (i < 0 || v < ov || (!(ov < v) && oi < i)) {
+        v = ov;
+        i = oi;
+    }
+}
+
+// the columns k .. k+19 (fewer at the end) of the factorisation; the running
+// diagonal values live in LDS when lds_piv (n * sizeof(T) <= 64 KB), else in
+// the global workspace
+template <typename T>
+__global__ __launch_bounds__(kPsThreads) void pstrf_block_kernel(PsView<T> L, int64_t n, int64_t k, T* __restrict__ gpiv,
                                                                  int64_t* __restrict__ P, int64_t* __restrict__ rank,
                                                                  int64_t* __restrict__ done,
-                                                                 const double* __restrict__ eps_p) {
-    __shared__ T rv[kPsThreads];
-    __shared__ int64_t ri[kPsThreads];
+                                                                 const double* __restrict__ eps_p, int lds_piv) {
+    extern __shared__ unsigned char ps_dyn[];
+    __shared__ T wv[kPsThreads / 64];
+    __shared__ int wi[kPsThreads / 64];
+    __shared__ int s_pivot;
     if (*done != 0) return;
-    const int tid = threadIdx.x;
+    T* piv = lds_piv ? reinterpret_cast<T*>(ps_dyn) : gpiv;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const double eps = *eps_p;
     const int cur = (int)min((int64_t)kPsBlock, n - k);
     for (int j = 0; j < cur; ++j) {
         const int64_t c = k + j;
-        // running diagonal values: restarted at the block's first column
+        // running diagonal values (restarted at the block's first column),
+        // updated and scanned for the first largest in one pass
+        T bv = T(0);
+        int bi = -1;
         if (j == 0) {
-            for (int64_t r = k + tid; r < n; r += kPsThreads) piv[r] = L(r, r);
+            for (int64_t r = k + tid; r < n; r += kPsThreads) {
+                const T x = L(r, r);
+                piv[r] = x;
+                argmax_combine(bv, bi, x, (int)r);
+            }
         } else {
             for (int64_t r = c + tid; r < n; r += kPsThreads) {
-                const T x = L(r, c - 1);
-                piv[r] -= x * x;
+                const T y = L(r, c - 1);
+                const T x = piv[r] - y * y;
+                piv[r] = x;
+                argmax_combine(bv, bi, x, (int)r);
             }
         }
-        __syncthreads();
-        // first largest of piv[c..n) (std::max_element)
-        T bv = T(0);
-        int64_t bi = -1;
-        for (int64_t r = c + tid; r < n; r += kPsThreads) {
-            const T x = piv[r];
-            if (bi < 0 || bv < x) {
-                bv = x;
-                bi = r;
-            }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const T ov = __shfl_xor(bv, off);
+            const int oi = __shfl_xor(bi, off);
+            argmax_combine(bv, bi, ov, oi);
         }
-        rv[tid] = bv;
-        ri[tid] = bi;
-        __syncthreads();
-        for (int off = kPsThreads / 2; off > 0; off >>= 1) {
-            if (tid < off) {
-                const int64_t oi = ri[tid + off];
-                if (oi >= 0) {
-                    const int64_t mi = ri[tid];
-                    const T ov = rv[tid + off], mv = rv[tid];
-                    if (mi < 0 || mv < ov || (!(ov < mv) && oi < mi)) {
-                        rv[tid] = ov;
-                        ri[tid] = oi;
-                    }
-                }
-            }
-            __syncthreads();
+        if (lane == 0) {
+            wv[wave] = bv;
+            wi[wave] = bi;
         }
-        const int64_t p = ri[0];
         __syncthreads();
-        if (p != c) {
-            if (tid == 0) P[c] = p;
-            for (int64_t t = tid; t < n; t += kPsThreads) {
+        if (wave == 0) {
+            T v = lane < kPsThreads / 64 ? wv[lane] : T(0);
+            int i = lane < kPsThreads / 64 ? wi[lane] : -1;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const T ov = __shfl_xor(v, off);
+                const int oi = __shfl_xor(i, off);
+                argmax_combine(v, i, ov, oi);
+            }
+            if (lane == 0) s_pivot = i;
+        }
+        __syncthreads();
+        const int64_t p = s_pivot;
+        if (p != c) {  // p > c; the four pieces are disjoint
+            if (tid == 0) {
+                P[c] = p;
+                T x = L(c, c);
+                L(c, c) = L(p, p);
+                L(p, p) = x;
+                x = piv[c];
+                piv[c] = piv[p];
+                piv[p] = x;
+            }
+            // rows c and p in the block's own columns; the earlier blocks'
+            // columns take the block's swaps afterwards (pstrf_row_swaps_kernel)
+            for (int64_t t = k + tid; t < c; t += kPsThreads) {
                 const T x = L(c, t);
                 L(c, t) = L(p, t);
                 L(p, t) = x;
             }
-            __syncthreads();
-            for (int64_t t = tid; t < n; t += kPsThreads) {
+            for (int64_t t = c + 1 + tid; t < p; t += kPsThreads) {
+                const T x = L(t, c);
+                L(t, c) = L(p, t);
+                L(p, t) = x;
+            }
+            for (int64_t t = p + 1 + tid; t < n; t += kPsThreads) {
                 const T x = L(t, c);
                 L(t, c) = L(t, p);
                 L(t, p) = x;
-            }
-            if (tid == 0) {
-                const T x = piv[c];
-                piv[c] = piv[p];
-                piv[p] = x;
             }
             __syncthreads();
         }
@@ -153,21 +189,36 @@ __global__ __launch_bounds__(kPsThreads) void pstrf_block_kernel(PsView<T> L, in
             for (int t = 0; t < j; ++t) acc += L(r, k + t) * L(c, k + t);
             L(r, c) = (L(r, c) - acc) / d;
         }
-        for (int64_t t = c + 1 + tid; t < n; t += kPsThreads) L(c, t) = T(0);
         __syncthreads();
     }
 }
 
-// after a stop at rank r < n: the trailing square [r, n) x [r, n) is zero
+// the row swaps of block k .. k+cur-1, in order, on the columns before k (not
+// read while the block runs): one thread per column
 template <typename T>
-__global__ __launch_bounds__(256) void pstrf_clear_kernel(T* __restrict__ A, int64_t lda, int64_t n,
+__global__ __launch_bounds__(256) void pstrf_row_swaps_kernel(PsView<T> L, int64_t k, int cur,
+                                                             const int64_t* __restrict__ P) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= k) return;
+    for (int j = 0; j < cur; ++j) {
+        const int64_t c = k + j, p = P[c];
+        if (p == c) continue;
+        const T x = L(c, t);
+        L(c, t) = L(p, t);
+        L(p, t) = x;
+    }
+}
+
+// the reference's zeros: the triangle opposite the factor, and after a stop
+// at rank r < n the trailing square [r, n) x [r, n)
+template <typename T>
+__global__ __launch_bounds__(256) void pstrf_clear_kernel(T* __restrict__ A, int64_t lda, int64_t n, bool upper,
                                                          const int64_t* __restrict__ rank) {
     const int64_t r = *rank;
-    if (r >= n) return;
-    const int64_t w = n - r;
-    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < w * w; e += (int64_t)gridDim.x * 256) {
-        const int64_t i = r + e % w, j = r + e / w;
-        A[i + j * lda] = T(0);
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n * n; e += (int64_t)gridDim.x * 256) {
+        const int64_t i = e % n, j = e / n;
+        const bool other = upper ? i > j : i < j;
+        if (other || (i >= r && j >= r)) A[i + j * lda] = T(0);
     }
 }
 
@@ -187,14 +238,24 @@ rmh_status pstrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
     hipLaunchKernelGGL((pstrf_init_kernel<T>), dim3(1), dim3(kPsThreads), 0, s, L, n, piv_out, rank, done, eps);
     RMH_TRY(check_launch("pstrf_init_kernel"));
     if (n == 0) return RMH_STATUS_SUCCESS;
+    const size_t piv_bytes = (size_t)n * sizeof(T);
+    const int lds_piv = piv_bytes <= 64 * 1024 ? 1 : 0;
     for (int64_t k = 0; k < n; k += kPsBlock) {
-        hipLaunchKernelGGL((pstrf_block_kernel<T>), dim3(1), dim3(kPsThreads), 0, s, L, n, k, piv, piv_out, rank, done,
-                           eps);
+        hipLaunchKernelGGL((pstrf_block_kernel<T>), dim3(1), dim3(kPsThreads), lds_piv ? piv_bytes : 0, s, L, n, k, piv,
+                           piv_out, rank, done, eps, lds_piv);
         RMH_TRY(check_launch("pstrf_block_kernel"));
+        if (k > 0) {
+            const int cur = (int)std::min<int64_t>(kPsBlock, n - k);
+            hipLaunchKernelGGL((pstrf_row_swaps_kernel<T>), dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, L, k,
+                               cur, piv_out);
+            RMH_TRY(check_launch("pstrf_row_swaps_kernel"));
+        }
         const int64_t m2 = n - k - kPsBlock;
         if (m2 <= 0) continue;
-        // trailing square -= L21 L21^T (both triangles, as the reference's gemm)
+        // trailing triangle -= L21 L21^T (the reference updates the whole square;
+        // only the factor's triangle is read here)
         GemmDesc<T> g;
+        g.part = upper ? Part::Upper : Part::Lower;
         g.M = m2; g.N = m2; g.K = kPsBlock;
         g.alpha = T(-1); g.beta = T(1);
         if (!upper) {  // L21 = A[k+20:, k:k+20]
@@ -207,11 +268,11 @@ rmh_status pstrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
         g.lda = lda; g.ldb = lda;
         g.C = A + (k + kPsBlock) + (k + kPsBlock) * lda; g.ldc = lda;
         g.abort_flag = done;
-        RMH_TRY(launch_gemm<T>(g, pick_tile(g.M, g.N, Part::Full), s));
+        RMH_TRY(launch_gemm<T>(g, pick_tile(g.M, g.N, g.part), s));
     }
     const int64_t cells = n * n;
     const unsigned grid = (unsigned)std::min<int64_t>((cells + 255) / 256, 4096);
-    hipLaunchKernelGGL((pstrf_clear_kernel<T>), dim3(grid), dim3(256), 0, s, A, lda, n, rank);
+    hipLaunchKernelGGL((pstrf_clear_kernel<T>), dim3(grid), dim3(256), 0, s, A, lda, n, upper, rank);
     return check_launch("pstrf_clear_kernel");
 }
 
