@@ -60,6 +60,7 @@ typedef system_tag<true> left;
 typedef system_tag<false> right;
 
 struct symm_pos_def {};
+struct symm_semi_pos_def {};
 
 // -------------------------------------------------------------- errors --
 namespace hip {

@@ -74,6 +74,14 @@ inline rmh_status potrf(rmh_handle h, int up, size_t n, double* A, size_t lda, i
 inline rmh_status potrf(rmh_handle h, int up, size_t n, float* A, size_t lda, int64_t* info, rmh_stream_t s) {
     return rmh_spotrf(h, up, n, A, lda, info, s);
 }
+inline rmh_status laswp(rmh_handle h, int inv, size_t m, size_t nc, double* B, size_t ldb, int rm, const int64_t* piv,
+                        rmh_stream_t s) {
+    return rmh_dlaswp(h, inv, m, nc, B, ldb, rm, piv, s);
+}
+inline rmh_status laswp(rmh_handle h, int inv, size_t m, size_t nc, float* B, size_t ldb, int rm, const int64_t* piv,
+                        rmh_stream_t s) {
+    return rmh_slaswp(h, inv, m, nc, B, ldb, rm, piv, s);
+}
 inline rmh_status pstrf(rmh_handle h, int up, size_t n, double* A, size_t lda, int64_t* piv, int64_t* rank,
                         rmh_stream_t s) {
     return rmh_dpstrf(h, up, n, A, lda, piv, rank, s);

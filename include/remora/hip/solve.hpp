@@ -91,6 +91,100 @@ struct matrix_temporary {
     typedef matrix<typename M::value_type, typename M::orientation, hip_tag> type;
 };
 
+// Solver for symmetric positive semi-definite systems on the pivoted
+// Cholesky factorisation P^T A P = L L^T of rank r
+// (symm_pos_semi_definite_solver, include/remora/decompositions.hpp:320-396):
+// at full rank two triangular solves; below it the least-squares solution
+// X = P L (L^T L)^-2 L^T P^T B with L the first r columns, (L^T L) factored
+// once.  The row interchanges run on the device (rmh_?laswp).
+template <class MatrixStorage>
+class symm_pos_semi_definite_solver {
+    typedef typename MatrixStorage::value_type value_type;
+
+public:
+    typedef hip_tag device_type;
+    template <class E>
+    explicit symm_pos_semi_definite_solver(matrix_expression<E, hip_tag> const& e)
+        : m_factor(e), m_piv(std::max<std::size_t>(e().size1(), 1), e().queue()) {
+        const std::size_t n = m_factor.size1();
+        hip::device& q = m_factor.queue();
+        q.set_device();
+        hip::stream& s = hip::get_stream(q);
+        auto sa = m_factor.raw_storage();
+        hip::buffer<int64_t> rank(1, q);
+        hip::check_rmh(hip::abi::pstrf(hip::get_handle(q), MatrixStorage::orientation::is_row_major, n, sa.values,
+                                       sa.leading_dimension, m_piv.get(), rank.get(), s.abi()));
+        int64_t r = 0;
+        hip::check_hip(hipMemcpyAsync(&r, rank.get(), sizeof(r), hipMemcpyDeviceToHost, s.handle()), "hipMemcpyAsync");
+        s.synchronize();
+        m_rank = std::size_t(r);
+        if (m_rank == n || m_rank == 0) return;
+        auto L = subrange(m_factor, 0, n, 0, m_rank);
+        MatrixStorage LtL(m_rank, m_rank, q);
+        clear(LtL);
+        kernels::gemm(trans(L), L, LtL, value_type(1));
+        m_cholesky.decompose(LtL);
+    }
+    std::size_t rank() const { return m_rank; }
+
+    template <class MatB>
+    void solve(matrix_expression<MatB, hip_tag>& B, left) const {
+        const std::size_t n = m_factor.size1();
+        swap_rows(B, false);
+        if (m_rank == 0) {
+            clear(B());
+        } else if (m_rank == n) {
+            kernels::trsm<lower, left>(m_factor, B);
+            auto Ut = trans(m_factor);
+            kernels::trsm<upper, left>(Ut, B);
+        } else {
+            auto L = subrange(m_factor, 0, n, 0, m_rank);
+            MatrixStorage Z(m_rank, B().size2(), m_factor.queue());
+            clear(Z);
+            kernels::gemm(trans(L), B(), Z, value_type(1));
+            m_cholesky.solve(Z, left());
+            m_cholesky.solve(Z, left());
+            clear(B());
+            kernels::gemm(L, Z, B, value_type(1));
+        }
+        swap_rows(B, true);
+    }
+    template <class MatB>
+    void solve(matrix_expression<MatB, hip_tag>& B, right) const {
+        auto Bt = trans(B);  // X A = B  <=>  A X^T = B^T (A symmetric)
+        solve(Bt, left());
+    }
+
+private:
+    template <class M>
+    void swap_rows(M& B, bool inverse) const {
+        auto sb = B.raw_storage();
+        hip::device& q = m_factor.queue();
+        hip::check_rmh(hip::abi::laswp(hip::get_handle(q), inverse, B.size1(), B.size2(), sb.values,
+                                       sb.leading_dimension, M::orientation::is_row_major, m_piv.get(),
+                                       hip::get_stream(q).abi()));
+    }
+    template <class M>
+    void swap_rows(matrix_expression<M, hip_tag>& B, bool inverse) const {
+        swap_rows(B(), inverse);
+    }
+    template <class M>
+    static void clear(M& X) {
+        if (X.size1() == 0 || X.size2() == 0) return;
+        auto sx = X.raw_storage();
+        const bool rm = M::orientation::is_row_major;
+        const std::size_t lines = rm ? X.size1() : X.size2(), len = rm ? X.size2() : X.size1();
+        hip::check_hip(hipMemset2DAsync(sx.values, sx.leading_dimension * sizeof(value_type), 0,
+                                        len * sizeof(value_type), lines, hip::get_stream(X.queue()).handle()),
+                       "hipMemset2DAsync");
+    }
+
+    MatrixStorage m_factor;
+    hip::buffer<int64_t> m_piv;
+    std::size_t m_rank = 0;
+    cholesky_decomposition<MatrixStorage> m_cholesky;
+};
+
 template <class MatA, class SolverType>
 struct solver;
 template <class MatA>
@@ -99,10 +193,17 @@ struct solver<MatA, symm_pos_def> : public cholesky_decomposition<typename matri
     solver(M const& m, symm_pos_def = symm_pos_def())
         : cholesky_decomposition<typename matrix_temporary<MatA>::type>(m) {}
 };
+template <class MatA>
+struct solver<MatA, symm_semi_pos_def>
+    : public symm_pos_semi_definite_solver<typename matrix_temporary<MatA>::type> {
+    template <class M>
+    solver(M const& m, symm_semi_pos_def = symm_semi_pos_def())
+        : symm_pos_semi_definite_solver<typename matrix_temporary<MatA>::type>(m) {}
+};
 
 // Lazy X = A^-1 B (left) or B A^-1 (right); evaluated on assignment to a
 // device matrix: X <- B, factor a copy of A, solve in place.
-template <class MatA, class MatB, class Side>
+template <class MatA, class MatB, class Side, class Tag = symm_pos_def>
 class matrix_matrix_solve {
 public:
     typedef typename MatB::value_type value_type;
@@ -113,7 +214,7 @@ public:
     template <class MatX>
     void assign_to(MatX& X) const {
         X = m_B;
-        solver<MatA, symm_pos_def> alg(m_A);
+        solver<MatA, Tag> alg(m_A);
         alg.solve(X, Side());
     }
     template <class O>
@@ -136,22 +237,38 @@ matrix_matrix_solve<MatA, MatB, Side> solve(matrix_expression<MatA, hip_tag> con
     return matrix_matrix_solve<MatA, MatB, Side>(A(), B());
 }
 
-// inv(A, symm_pos_def()) is only usable inside prod(), which rewrites to a solve
-template <class MatA>
+template <class MatA, class MatB, class Side>
+matrix_matrix_solve<MatA, MatB, Side, symm_semi_pos_def> solve(matrix_expression<MatA, hip_tag> const& A,
+                                                               matrix_expression<MatB, hip_tag> const& B,
+                                                               symm_semi_pos_def, Side) {
+    assert(A().size1() == A().size2());
+    assert(Side::is_left ? A().size2() == B().size1() : A().size1() == B().size2());
+    return matrix_matrix_solve<MatA, MatB, Side, symm_semi_pos_def>(A(), B());
+}
+
+// inv(A, symm_pos_def()) / inv(A, symm_semi_pos_def()) are only usable inside
+// prod(), which rewrites to a solve
+template <class MatA, class Tag = symm_pos_def>
 struct matrix_inverse {
     MatA const& A;
 };
 template <class MatA>
-matrix_inverse<MatA> inv(matrix_expression<MatA, hip_tag> const& A, symm_pos_def) {
+matrix_inverse<MatA, symm_pos_def> inv(matrix_expression<MatA, hip_tag> const& A, symm_pos_def) {
     return {A()};
 }
-template <class MatA, class MatB>
-matrix_matrix_solve<MatA, MatB, left> prod(matrix_inverse<MatA> const& Ainv, matrix_expression<MatB, hip_tag> const& B) {
-    return matrix_matrix_solve<MatA, MatB, left>(Ainv.A, B());
+template <class MatA>
+matrix_inverse<MatA, symm_semi_pos_def> inv(matrix_expression<MatA, hip_tag> const& A, symm_semi_pos_def) {
+    return {A()};
 }
-template <class MatB, class MatA>
-matrix_matrix_solve<MatA, MatB, right> prod(matrix_expression<MatB, hip_tag> const& B, matrix_inverse<MatA> const& Ainv) {
-    return matrix_matrix_solve<MatA, MatB, right>(Ainv.A, B());
+template <class MatA, class Tag, class MatB>
+matrix_matrix_solve<MatA, MatB, left, Tag> prod(matrix_inverse<MatA, Tag> const& Ainv,
+                                                matrix_expression<MatB, hip_tag> const& B) {
+    return matrix_matrix_solve<MatA, MatB, left, Tag>(Ainv.A, B());
+}
+template <class MatB, class MatA, class Tag>
+matrix_matrix_solve<MatA, MatB, right, Tag> prod(matrix_expression<MatB, hip_tag> const& B,
+                                                 matrix_inverse<MatA, Tag> const& Ainv) {
+    return matrix_matrix_solve<MatA, MatB, right, Tag>(Ainv.A, B());
 }
 
 }  // namespace remora

@@ -132,6 +132,17 @@ rmh_status rmh_dpstrf(rmh_handle h, int upper, size_t n, double* A, size_t lda, 
 rmh_status rmh_spstrf(rmh_handle h, int upper, size_t n, float* A, size_t lda, int64_t* d_piv, int64_t* d_rank,
                       rmh_stream_t stream);
 
+/* Row interchanges of rmh_?pstrf's d_piv on an m x ncols matrix B (column-
+ * major with leading dimension ldb, or row-major when row_major): rows j and
+ * d_piv[j] are swapped for j = 0 .. m-1, or m-1 .. 0 when `inverse`.  These
+ * are the reference's swap_rows / swap_rows_inverted with a permutation_matrix
+ * (used by symm_pos_semi_definite_solver::solve,
+ * include/remora/decompositions.hpp:360-375).  Asynchronous. */
+rmh_status rmh_dlaswp(rmh_handle h, int inverse, size_t m, size_t ncols, double* B, size_t ldb, int row_major,
+                      const int64_t* d_piv, rmh_stream_t stream);
+rmh_status rmh_slaswp(rmh_handle h, int inverse, size_t m, size_t ncols, float* B, size_t ldb, int row_major,
+                      const int64_t* d_piv, rmh_stream_t stream);
+
 /* Rank-one update of a Cholesky factor in place: with A = L L^T on entry
  * (the factor from rmh_?potrf: L lower, or U = L^T upper; only that triangle
  * read and written), on return L L^T = alpha*A + beta*v*v^T.  Replaces

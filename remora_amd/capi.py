@@ -65,6 +65,8 @@ SIGNATURES = {
     "rmh_dpotrs": (ST, [P, I, S, S, P, S, P, S, P]),
     "rmh_spotrs": (ST, [P, I, S, S, P, S, P, S, P]),
     "rmh_dsposv": (ST, [P, I, I, S, S, P, S, P, S, P, S, D, I, C.POINTER(I), C.POINTER(D), P]),
+    "rmh_dlaswp": (ST, [P, I, S, S, P, S, I, P, P]),
+    "rmh_slaswp": (ST, [P, I, S, S, P, S, I, P, P]),
     "rmh_dpstrf": (ST, [P, I, S, P, S, P, P, P]),
     "rmh_spstrf": (ST, [P, I, S, P, S, P, P, P]),
     "rmh_dchol_update": (ST, [P, I, S, D, D, P, S, P, S, P, P]),

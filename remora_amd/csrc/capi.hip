@@ -726,6 +726,22 @@ rmh_status rmh_spstrf(rmh_handle h, int upper, size_t n, float* A, size_t lda, i
                       rmh_stream_t stream) {
     return pstrf_entry<float>(h, upper, n, A, lda, d_piv, d_rank, stream);
 }
+rmh_status rmh_dlaswp(rmh_handle h, int inverse, size_t m, size_t ncols, double* B, size_t ldb, int row_major,
+                      const int64_t* d_piv, rmh_stream_t stream) {
+    RMH_CHECK_HANDLE(h);
+    RMH_REQUIRE(ldb >= mx1(row_major ? ncols : m), "laswp: ldb too small");
+    RMH_REQUIRE(m == 0 || ncols == 0 || (B != nullptr && d_piv != nullptr), "laswp: NULL pointer");
+    return laswp_impl<double>(inverse != 0, (int64_t)m, (int64_t)ncols, B, (int64_t)ldb, row_major != 0, d_piv,
+                              hs(stream));
+}
+rmh_status rmh_slaswp(rmh_handle h, int inverse, size_t m, size_t ncols, float* B, size_t ldb, int row_major,
+                      const int64_t* d_piv, rmh_stream_t stream) {
+    RMH_CHECK_HANDLE(h);
+    RMH_REQUIRE(ldb >= mx1(row_major ? ncols : m), "laswp: ldb too small");
+    RMH_REQUIRE(m == 0 || ncols == 0 || (B != nullptr && d_piv != nullptr), "laswp: NULL pointer");
+    return laswp_impl<float>(inverse != 0, (int64_t)m, (int64_t)ncols, B, (int64_t)ldb, row_major != 0, d_piv,
+                             hs(stream));
+}
 rmh_status rmh_dchol_update(rmh_handle h, int upper, size_t n, double alpha, double beta, double* L, size_t ldl,
                             const double* v, size_t incv, int64_t* d_info, rmh_stream_t stream) {
     return chol_update_entry<double>(h, upper, n, alpha, beta, L, ldl, v, incv, d_info, stream);

@@ -222,7 +222,36 @@ __global__ __launch_bounds__(256) void pstrf_clear_kernel(T* __restrict__ A, int
     }
 }
 
+// row interchanges of a pivoted factorisation on an m x ncols matrix B,
+// element (i,j) at B[i*rs + j*cs]: swap rows j and piv[j] for j = 0 .. m-1
+// (swap_rows) or m-1 .. 0 (swap_rows_inverted); one thread per column
+template <typename T>
+__global__ __launch_bounds__(256) void laswp_kernel(T* __restrict__ B, int64_t rs, int64_t cs, int64_t m,
+                                                   int64_t ncols, const int64_t* __restrict__ piv, bool inverse) {
+    const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (col >= ncols) return;
+    for (int64_t q = 0; q < m; ++q) {
+        const int64_t j = inverse ? m - 1 - q : q, p = piv[j];
+        if (p == j || p < 0 || p >= m) continue;
+        const T x = B[j * rs + col * cs];
+        B[j * rs + col * cs] = B[p * rs + col * cs];
+        B[p * rs + col * cs] = x;
+    }
+}
+
 }  // namespace
+
+template <typename T>
+rmh_status laswp_impl(bool inverse, int64_t m, int64_t ncols, T* B, int64_t ldb, bool row_major, const int64_t* piv,
+                      hipStream_t s) {
+    if (m == 0 || ncols == 0) return RMH_STATUS_SUCCESS;
+    const int64_t rs = row_major ? ldb : 1, cs = row_major ? 1 : ldb;
+    hipLaunchKernelGGL((laswp_kernel<T>), dim3((unsigned)((ncols + 255) / 256)), dim3(256), 0, s, B, rs, cs, m, ncols,
+                       piv, inverse);
+    return check_launch("laswp_kernel");
+}
+template rmh_status laswp_impl<double>(bool, int64_t, int64_t, double*, int64_t, bool, const int64_t*, hipStream_t);
+template rmh_status laswp_impl<float>(bool, int64_t, int64_t, float*, int64_t, bool, const int64_t*, hipStream_t);
 
 template <typename T>
 rmh_status pstrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int64_t* piv_out, int64_t* rank,

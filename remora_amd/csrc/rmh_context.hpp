@@ -118,6 +118,9 @@ rmh_status potrf_adjoint_impl(Context* h, bool upper, int64_t n, const T* L, int
                               T* Abar, int64_t ldab, hipStream_t s);
 
 template <typename T>
+rmh_status laswp_impl(bool inverse, int64_t m, int64_t ncols, T* B, int64_t ldb, bool row_major, const int64_t* piv,
+                      hipStream_t s);
+template <typename T>
 rmh_status pstrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int64_t* piv, int64_t* rank, hipStream_t s);
 
 template <typename T>

@@ -320,6 +320,82 @@ static void pstrf_test() {
     }
 }
 
+// solve(A, B, symm_semi_pos_def(), left|right) (Test/symm_solve.cpp:190-230):
+// full-rank createSymm(128), B(i,j) = 0.1 j + i/128 with 151 columns: A X = B
+// to 1e-12 and solve == prod(inv(A, symm_semi_pos_def()), B).  Then a rank-50
+// consistent system A = R R^T, B = A Y, whose least-squares solution
+// satisfies A X = B.
+template <class O>
+static void semi_solve_test() {
+    const std::size_t n = 128, k = 151;
+    auto A = create_symm<row_major>(n, false);
+    matrix<double, O, cpu_tag> B(n, k), Bright(k, n);
+    for (std::size_t i = 0; i != n; ++i)
+        for (std::size_t j = 0; j != k; ++j) Bright(j, i) = B(i, j) = 0.1 * j + (1.0 / n) * i;
+    auto dA = copy_to_device(A, hip_tag());
+    auto dB = copy_to_device(B, hip_tag());
+    auto dBr = copy_to_device(Bright, hip_tag());
+    matrix<double, O, hip_tag> dX = solve(dA, dB, symm_semi_pos_def(), left());
+    matrix<double, O, hip_tag> dXp = prod(inv(dA, symm_semi_pos_def()), dB);
+    matrix<double, O, hip_tag> dXr = solve(dA, dBr, symm_semi_pos_def(), right());
+    auto X = copy_to_cpu(dX), Xp = copy_to_cpu(dXp), Xr = copy_to_cpu(dXr);
+    double err = 0, errr = 0, same = 0;
+    for (std::size_t i = 0; i != n; ++i)
+        for (std::size_t j = 0; j != k; ++j) {
+            double s = 0, sr = 0;
+            for (std::size_t t = 0; t != n; ++t) {
+                s += A(i, t) * X(t, j);
+                sr += Xr(j, t) * A(t, i);
+            }
+            err = std::fmax(err, std::fabs(s - B(i, j)));
+            errr = std::fmax(errr, std::fabs(sr - Bright(j, i)));
+            same = std::fmax(same, std::fabs(X(i, j) - Xp(i, j)));
+        }
+    CHECK_SMALL(err, 1e-12);
+    CHECK_SMALL(errr, 1e-12);
+    CHECK_SMALL(same, 1e-15);
+    // rank 50
+    const std::size_t r = 50, m = 3;
+    matrix<double, row_major, cpu_tag> Ad(n, n), Y(n, m);
+    matrix<double, O, cpu_tag> Bd(n, m);
+    std::vector<double> R(n * r);
+    for (std::size_t i = 0; i != n; ++i)
+        for (std::size_t t = 0; t != r; ++t) {  // pseudo-random entries in [-0.5, 0.5): rank r
+            const double h = std::sin(12.9898 * i + 78.233 * t + 0.1) * 43758.5453;
+            R[i * r + t] = h - std::floor(h) - 0.5;
+        }
+    for (std::size_t i = 0; i != n; ++i)
+        for (std::size_t j = 0; j != n; ++j) {
+            double s = 0;
+            for (std::size_t t = 0; t != r; ++t) s += R[i * r + t] * R[j * r + t];
+            Ad(i, j) = s;
+        }
+    for (std::size_t i = 0; i != n; ++i)
+        for (std::size_t j = 0; j != m; ++j) Y(i, j) = std::cos(0.11 * i + 0.7 * j);
+    double bmax = 0;
+    for (std::size_t i = 0; i != n; ++i)
+        for (std::size_t j = 0; j != m; ++j) {
+            double s = 0;
+            for (std::size_t t = 0; t != n; ++t) s += Ad(i, t) * Y(t, j);
+            Bd(i, j) = s;
+            bmax = std::fmax(bmax, std::fabs(s));
+        }
+    auto dAd = copy_to_device(Ad, hip_tag());
+    auto dBd = copy_to_device(Bd, hip_tag());
+    solver<matrix<double, row_major, hip_tag>, symm_semi_pos_def> alg(dAd);
+    CHECK_SMALL(double(alg.rank()) - double(r), 0.0);
+    matrix<double, O, hip_tag> dXd = solve(dAd, dBd, symm_semi_pos_def(), left());
+    auto Xd = copy_to_cpu(dXd);
+    double errd = 0;
+    for (std::size_t i = 0; i != n; ++i)
+        for (std::size_t j = 0; j != m; ++j) {
+            double s = 0;
+            for (std::size_t t = 0; t != n; ++t) s += Ad(i, t) * Xd(t, j);
+            errd = std::fmax(errd, std::fabs(s - Bd(i, j)));
+        }
+    CHECK_SMALL(errd / bmax, 1e-10);
+}
+
 // copy_to_device / copy_to_cpu round trip exact, above the staging threshold
 // with a ragged last chunk (3000 x 3001 doubles = 72 MB: two 64 MB chunks),
 // and below it.
@@ -351,6 +427,8 @@ static void group(const char* name, void (*fn)()) {
 
 int main() {
     std::setvbuf(stdout, nullptr, _IONBF, 0);
+    group("semi-definite solve row_major", semi_solve_test<row_major>);
+    group("semi-definite solve column_major", semi_solve_test<column_major>);
     group("pstrf row_major", pstrf_test<row_major>);
     group("pstrf column_major", pstrf_test<column_major>);
     group("transport row_major", transport_test<row_major>);

@@ -108,3 +108,32 @@ def test_pstrf_empty():
     piv, rank = la.pstrf(handle(), A)
     torch.cuda.synchronize()
     assert int(rank.item()) == 0
+
+
+@pytest.mark.parametrize("row_major", [False, True])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_laswp_matches_transpositions(inverse, row_major):
+    """rmh_dlaswp applies d_piv's transpositions to the rows of B in order
+    (swap_rows) or in reverse (swap_rows_inverted), either storage order."""
+    import ctypes as C
+    m, k = 37, 5
+    rng = np.random.default_rng(2)
+    piv = np.array([rng.integers(j, m) for j in range(m)], dtype=np.int64)
+    B = rng.standard_normal((m, k))
+    ref = B.copy()
+    order = range(m - 1, -1, -1) if inverse else range(m)
+    for j in order:
+        ref[[j, piv[j]], :] = ref[[piv[j], j], :]
+    if row_major:
+        Bd = torch.from_numpy(B.copy()).cuda()  # row-major m x k, ld k
+        ld = k
+    else:
+        Bd = torch.from_numpy(np.asfortranarray(B).T.copy()).cuda()  # column-major storage, ld m
+        ld = m
+    pd = torch.from_numpy(piv).cuda()
+    st = capi.load().rmh_dlaswp(handle().value, int(inverse), m, k, C.c_void_p(Bd.data_ptr()), ld, int(row_major),
+                                C.c_void_p(pd.data_ptr()), None)
+    assert st == capi.SUCCESS
+    torch.cuda.synchronize()
+    got = Bd.cpu().numpy() if row_major else Bd.cpu().numpy().T
+    assert np.array_equal(got, ref)
