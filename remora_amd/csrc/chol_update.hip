@@ -12,8 +12,8 @@
 // then a_j-scaled old entry plus c_j times the running row sum
 // v_i - sum_{k<=j} g_k * a*L(i,k), a scan along each row.  On the device:
 //   1. p = L^-1 v on the blocked TRSM;
-//   2. one workgroup computes every column's scalars (and the first column
-//      whose update would make the matrix indefinite);
+//   2. every column's scalars (and the first column whose update would make
+//      the matrix indefinite), the prefix sum taken over 256-column blocks;
 //   3. the strictly lower 256 x 256 tiles write their rows' partial sums;
 //   4. every lower tile seeds its rows' running sums from the partial sums of
 //      the tiles to its left and rewrites its entries in place.
@@ -41,62 +41,111 @@ __device__ __forceinline__ void tri_tile(int64_t t, int64_t& r, int64_t& c) {
 }
 
 // Column scalars of the sweep, with the reference's formulas; beta' is the
-// exclusive prefix sum 1 + sum_{k<j} beta*w_k^2/Ljj_k^2.  flag_j = 1 when
-// gamma_j == 0 (the reference only rescales that column by a).
+// exclusive prefix sum 1 + sum_{k<j} beta*w_k^2/Ljj_k^2, taken in three
+// launches over 256-column blocks (terms and block sums, a scan of the block
+// sums, the in-block scan and the scalars).  flag_j = 1 when gamma_j == 0
+// (the reference only rescales that column by a).
+constexpr int kScanBlock = 256;
+
 template <typename T>
-__global__ __launch_bounds__(kScalarThreads) void update_scalars_kernel(const T* __restrict__ L, int64_t ldl, int64_t n,
-                                                                       const T* __restrict__ p, double alpha, double beta,
-                                                                       double* __restrict__ g, double* __restrict__ r,
-                                                                       double* __restrict__ c, double* __restrict__ dnew,
-                                                                       int* __restrict__ flag, int64_t* __restrict__ info) {
-    __shared__ double part[kScalarThreads];
-    __shared__ unsigned long long first_bad;
-    const int t = threadIdx.x;
-    if (t == 0) first_bad = ~0ull;
-    const double a = sqrt(alpha);
-    const int64_t chunk = (n + kScalarThreads - 1) / kScalarThreads;
-    const int64_t j0 = min(n, (int64_t)t * chunk), j1 = min(n, j0 + chunk);
-    auto term = [&](int64_t j, double& Ljj, double& dj, double& wj, double& swj2) {
+struct ColumnTerm {
+    double Ljj, dj, wj, swj2;
+    __device__ ColumnTerm(const T* L, int64_t ldl, const T* p, int64_t j, double a, double beta) {
         const double d = (double)L[j + j * ldl];
         Ljj = a * d;
         dj = Ljj * Ljj;
         wj = d * (double)p[j];
         swj2 = beta * wj * wj;
-    };
-    double sum = 0.0;
-#pragma unroll 8
-    for (int64_t j = j0; j < j1; ++j) {
-        double Ljj, dj, wj, swj2;
-        term(j, Ljj, dj, wj, swj2);
-        sum += swj2 / dj;
     }
+};
+
+template <typename T>
+__global__ __launch_bounds__(kScanBlock) void update_terms_kernel(const T* __restrict__ L, int64_t ldl, int64_t n,
+                                                                  const T* __restrict__ p, double alpha, double beta,
+                                                                  double* __restrict__ term, double* __restrict__ bsum) {
+    __shared__ double red[kScanBlock];
+    const int t = threadIdx.x;
+    const int64_t j = (int64_t)blockIdx.x * kScanBlock + t;
+    double x = 0.0;
+    if (j < n) {
+        const ColumnTerm<T> c(L, ldl, p, j, sqrt(alpha), beta);
+        x = c.swj2 / c.dj;
+        term[j] = x;
+    }
+    red[t] = x;
+    __syncthreads();
+    for (int off = kScanBlock / 2; off > 0; off >>= 1) {
+        if (t < off) red[t] += red[t + off];
+        __syncthreads();
+    }
+    if (t == 0) bsum[blockIdx.x] = red[0];
+}
+
+// exclusive scan of the nb block sums in place (one workgroup), and the
+// first-failure marker reset
+__global__ __launch_bounds__(kScalarThreads) void update_block_scan_kernel(double* __restrict__ bsum, int64_t nb,
+                                                                           unsigned long long* __restrict__ first_bad) {
+    __shared__ double part[kScalarThreads];
+    const int t = threadIdx.x;
+    const int64_t chunk = (nb + kScalarThreads - 1) / kScalarThreads;
+    const int64_t b0 = min(nb, (int64_t)t * chunk), b1 = min(nb, b0 + chunk);
+    double sum = 0.0;
+    for (int64_t b = b0; b < b1; ++b) sum += bsum[b];
     part[t] = sum;
     __syncthreads();
-    // inclusive scan of the per-thread sums (Hillis-Steele)
     for (int off = 1; off < kScalarThreads; off <<= 1) {
         const double add = t >= off ? part[t - off] : 0.0;
         __syncthreads();
         part[t] += add;
         __syncthreads();
     }
-    double bp = 1.0 + (t > 0 ? part[t - 1] : 0.0);
-#pragma unroll 8
-    for (int64_t j = j0; j < j1; ++j) {
-        double Ljj, dj, wj, swj2;
-        term(j, Ljj, dj, wj, swj2);
-        const double gamma = dj * bp + swj2;
-        const double x = dj + swj2 / bp;
-        if (x <= 0.0) atomicMin(&first_bad, (unsigned long long)j);
-        const double nLjj = sqrt(x);
-        bp += swj2 / dj;
-        g[j] = wj / Ljj;
-        r[j] = nLjj / Ljj;
-        c[j] = nLjj * beta * wj / gamma;
-        dnew[j] = nLjj;
-        flag[j] = gamma == 0.0 ? 1 : 0;
+    double run = t > 0 ? part[t - 1] : 0.0;
+    for (int64_t b = b0; b < b1; ++b) {
+        const double x = bsum[b];
+        bsum[b] = run;
+        run += x;
     }
+    if (t == 0) *first_bad = ~0ull;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kScanBlock) void update_scalars_kernel(const T* __restrict__ L, int64_t ldl, int64_t n,
+                                                                    const T* __restrict__ p, double alpha, double beta,
+                                                                    const double* __restrict__ term,
+                                                                    const double* __restrict__ bpre,
+                                                                    double* __restrict__ g, double* __restrict__ r,
+                                                                    double* __restrict__ c, double* __restrict__ dnew,
+                                                                    int* __restrict__ flag,
+                                                                    unsigned long long* __restrict__ first_bad) {
+    __shared__ double sc[kScanBlock];
+    const int t = threadIdx.x;
+    const int64_t j = (int64_t)blockIdx.x * kScanBlock + t;
+    const double x0 = j < n ? term[j] : 0.0;
+    sc[t] = x0;
     __syncthreads();
-    if (t == 0) *info = first_bad == ~0ull ? 0 : (int64_t)first_bad + 1;
+    for (int off = 1; off < kScanBlock; off <<= 1) {  // inclusive scan
+        const double add = t >= off ? sc[t - off] : 0.0;
+        __syncthreads();
+        sc[t] += add;
+        __syncthreads();
+    }
+    if (j >= n) return;
+    const double bp = 1.0 + bpre[blockIdx.x] + (sc[t] - x0);
+    const ColumnTerm<T> cj(L, ldl, p, j, sqrt(alpha), beta);
+    const double gamma = cj.dj * bp + cj.swj2;
+    const double x = cj.dj + cj.swj2 / bp;
+    if (x <= 0.0) atomicMin(first_bad, (unsigned long long)j);
+    const double nLjj = sqrt(x);
+    g[j] = cj.wj / cj.Ljj;
+    r[j] = nLjj / cj.Ljj;
+    c[j] = nLjj * beta * cj.wj / gamma;
+    dnew[j] = nLjj;
+    flag[j] = gamma == 0.0 ? 1 : 0;
+}
+
+__global__ void update_info_kernel(const unsigned long long* __restrict__ first_bad, int64_t* __restrict__ info) {
+    const unsigned long long f = *first_bad;
+    *info = f == ~0ull ? 0 : (int64_t)f + 1;
 }
 
 // One 256 x 256 tile of the triangle.  L(i,k) is at L[i + k*ldl] (lower
@@ -203,8 +252,10 @@ rmh_status chol_update_impl(Context* h, bool upper, int64_t n, T alpha, T beta, 
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };
     const size_t vb = al((size_t)n * sizeof(T)), db = al((size_t)n * sizeof(double)), fb = al((size_t)n * sizeof(int));
     const size_t sb = al((size_t)std::max<int64_t>(nblk - 1, 1) * n * sizeof(double));
+    const int64_t nsb = (n + kScanBlock - 1) / kScanBlock;
+    const size_t bb = al((size_t)nsb * sizeof(double)) + 256;
     void* base = nullptr;
-    RMH_TRY(h->reserve_slot(WS_UPDATE, 2 * vb + 4 * db + fb + sb, &base));
+    RMH_TRY(h->reserve_slot(WS_UPDATE, 2 * vb + 5 * db + fb + sb + bb, &base));
     char* w = static_cast<char*>(base);
     T* vin = reinterpret_cast<T*>(w); w += vb;
     T* p = reinterpret_cast<T*>(w); w += vb;
@@ -213,15 +264,25 @@ rmh_status chol_update_impl(Context* h, bool upper, int64_t n, T alpha, T beta, 
     double* c = reinterpret_cast<double*>(w); w += db;
     double* dnew = reinterpret_cast<double*>(w); w += db;
     int* flag = reinterpret_cast<int*>(w); w += fb;
-    double* S = reinterpret_cast<double*>(w);
+    double* S = reinterpret_cast<double*>(w); w += sb;
+    double* term = reinterpret_cast<double*>(w); w += db;
+    double* bsum = reinterpret_cast<double*>(w); w += al((size_t)nsb * sizeof(double));
+    unsigned long long* first_bad = reinterpret_cast<unsigned long long*>(w);
 
     RMH_TRY(launch_strided_copy<T>(v, incv, vin, 1, n, s));
     RMH_TRY(launch_strided_copy<T>(v, incv, p, 1, n, s));
     // p = L^-1 v (lower storage) or U^-T v (upper storage)
     RMH_TRY(trsm_impl<T>(h, true, upper, upper, false, n, 1, T(1), L, ldl, p, n, s));
-    hipLaunchKernelGGL((update_scalars_kernel<T>), dim3(1), dim3(kScalarThreads), 0, s, L, ldl, n, p, (double)alpha,
-                       (double)beta, g, r, c, dnew, flag, info);
+    hipLaunchKernelGGL((update_terms_kernel<T>), dim3((unsigned)nsb), dim3(kScanBlock), 0, s, L, ldl, n, p,
+                       (double)alpha, (double)beta, term, bsum);
+    RMH_TRY(check_launch("update_terms_kernel"));
+    hipLaunchKernelGGL(update_block_scan_kernel, dim3(1), dim3(kScalarThreads), 0, s, bsum, nsb, first_bad);
+    RMH_TRY(check_launch("update_block_scan_kernel"));
+    hipLaunchKernelGGL((update_scalars_kernel<T>), dim3((unsigned)nsb), dim3(kScanBlock), 0, s, L, ldl, n, p,
+                       (double)alpha, (double)beta, term, bsum, g, r, c, dnew, flag, first_bad);
     RMH_TRY(check_launch("update_scalars_kernel"));
+    hipLaunchKernelGGL(update_info_kernel, dim3(1), dim3(1), 0, s, first_bad, info);
+    RMH_TRY(check_launch("update_info_kernel"));
     const double a = std::sqrt((double)alpha);
     if (nblk > 1) {
         const unsigned tiles = (unsigned)((nblk - 1) * nblk / 2);
