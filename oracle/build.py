@@ -29,7 +29,8 @@ LIB = os.path.join(HERE, "liboracle.so")
 CFLAGS = ["-O3", "-march=x86-64-v3", "-DREMORA_USE_SIMD", "-DNDEBUG", "-std=gnu11",
           "-fPIC", "-shared", "-Wall", "-Wno-unused-function"]
 
-MUTANTS = ("UGEMM_DROP_LAST_K", "TRSM_TRANSPOSED_A", "POTRF_SIGN", "SYRK_WRONG_TRIANGLE", "UPDATE_UNSCALED_TEMP", "PSTRF_LAST_MAX")
+MUTANTS = ("UGEMM_DROP_LAST_K", "TRSM_TRANSPOSED_A", "POTRF_SIGN", "SYRK_WRONG_TRIANGLE", "UPDATE_UNSCALED_TEMP", "PSTRF_LAST_MAX",
+           "TRSV_SKIP_FIRST", "GEMV_NO_ALPHA")
 
 
 def build(out: str = LIB, mutant: str | None = None, verbose: bool = False) -> str:

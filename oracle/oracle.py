@@ -51,6 +51,13 @@ def _declare(lib):
     lib.orc_dchol_update.restype = S
     lib.orc_dpstrf.argtypes = [P, S, S, I, I, P]
     lib.orc_dpstrf.restype = S
+    Q = C.c_ssize_t
+    lib.orc_dtrsv.argtypes = [P, S, S, I, P, Q, I, I, I]
+    lib.orc_dtrsv.restype = I
+    lib.orc_dgemv.argtypes = [P, S, S, S, I, P, Q, P, Q, D]
+    lib.orc_dgemv.restype = None
+    lib.orc_dposv_vec.argtypes = [P, S, S, I, P, Q, C.POINTER(S)]
+    lib.orc_dposv_vec.restype = I
     lib.orc_vector_length.restype = I
     lib.orc_simd_enabled.restype = I
 
@@ -180,6 +187,56 @@ def swap_full_inverted(P, M):
             M[[i, p], :] = M[[p, i], :]
             M[:, [i, p]] = M[:, [p, i]]
     return M
+
+
+def _strided(v, inc: int) -> np.ndarray:
+    """A float64 buffer holding v at stride inc (gaps filled with NaN)."""
+    v = np.asarray(v, dtype=np.float64).ravel()
+    buf = np.full(max(1, (len(v) - 1) * inc + 1), np.nan)
+    buf[::inc] = v
+    return buf
+
+
+def trsv(A, b, upper: bool, unit: bool, left: bool = True, a_row_major=True, inc: int = 1, lib=None):
+    """kernels::trsv<Triangular, Side>(A, b) (kernels/trsv.hpp:53-63 ->
+    default/trsv.hpp:44-183) on a copy of b stored at stride inc.
+
+    Returns (x, singular): singular mirrors the reference's
+    std::invalid_argument on a zero diagonal."""
+    lib = lib or load()
+    a = _prep(A, a_row_major)
+    n = a.shape[0]
+    buf = _strided(b, inc)
+    sing = lib.orc_dtrsv(_ptr(a), n, max(_ld(a, a_row_major), 1) if n else 1, int(a_row_major), _ptr(buf), inc,
+                         int(upper), int(unit), int(left))
+    return buf[::inc][:n].copy(), bool(sing)
+
+
+def gemv(A, x, y, alpha: float, a_row_major=True, lib=None):
+    """kernels::gemv(A, x, y, alpha): y += alpha A x (kernels/gemv.hpp:58-72
+    -> default/gemv.hpp:43-93).  Returns the updated copy of y."""
+    lib = lib or load()
+    a = _prep(A, a_row_major)
+    m, n = a.shape
+    xx = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    yy = np.array(y, dtype=np.float64, copy=True)
+    lib.orc_dgemv(_ptr(a), m, n, max(_ld(a, a_row_major), 1) if m and n else 1, int(a_row_major), _ptr(xx), 1,
+                  _ptr(yy), 1, float(alpha))
+    return yy
+
+
+def posv_vec(A, b, a_row_major=True, lib=None):
+    """solve(A, b, symm_pos_def(), left|right) for a vector b through
+    cholesky_decomposition (solve.hpp:90-101, decompositions.hpp:81-91,
+    111-115).  Returns (x, L_with_upper_of_A, info, singular)."""
+    lib = lib or load()
+    a = _prep(A, a_row_major)
+    n = a.shape[0]
+    bb = np.array(b, dtype=np.float64, copy=True).ravel()
+    info = C.c_size_t(0)
+    sing = lib.orc_dposv_vec(_ptr(a), n, max(_ld(a, a_row_major), 1) if n else 1, int(a_row_major), _ptr(bb), 1,
+                             C.byref(info))
+    return bb, a, int(info.value), bool(sing)
 
 
 def vector_length(lib=None) -> int:

@@ -18,6 +18,10 @@
  *   cholesky_decomposition::solve ...... include/remora/decompositions.hpp:101-115
  *   cholesky_decomposition::update ..... include/remora/decompositions.hpp:132-169
  *   pstrf (pivoted, block 20) .......... include/remora/kernels/default/pstrf.hpp:40-160
+ *   trsv_impl (4 variants + right) ..... include/remora/kernels/default/trsv.hpp:44-183
+ *   gemv_impl (row/column-major) ....... include/remora/kernels/default/gemv.hpp:43-93
+ *   dot (dense) ........................ include/remora/kernels/default/dot.hpp:39-52
+ *   cholesky_decomposition::solve(b) ... include/remora/decompositions.hpp:111-115
  *
  * Matrices are strided views: element (i,j) lives at p[i*s1 + j*s2] and the
  * view carries Remora's orientation tag (row_major: s2 is the unit stride).
@@ -624,6 +628,162 @@ size_t orc_dpstrf(double *Ap, size_t m, size_t lda, int row_major, int upper, in
     }
     free(pv);
     return m;
+}
+
+/* ============================================================ trsv.hpp == */
+/* A dense vector of Remora is a strided array: element i at x[i*inc]. */
+static int g_trsv_singular = 0;
+
+/* dot(v1, v2) of default/dot.hpp:39-52: result starts at 0 and takes
+ * v1_k * v2_k in increasing k. */
+static double dot_strided(const double *v1, ptrdiff_t inc1, const double *v2, ptrdiff_t inc2, size_t len) {
+    double r = 0.0;
+    for (size_t k = 0; k < len; ++k) r += v1[(ptrdiff_t)k * inc1] * v2[(ptrdiff_t)k * inc2];
+    return r;
+}
+
+/* lower, column-major storage: b_n /= A_nn, then the column below takes
+ * b[n+1:] = b[n+1:] + (-b_n) * A[n+1:, n], skipped when b_n == 0
+ * (default/trsv.hpp:44-68; multiply_and_add is x + scalar * y,
+ * cpu/traits.hpp:131-140). */
+static void trsv_lower_cm(view A, double *b, ptrdiff_t inc, int unit) {
+    const size_t size = A.m;
+    for (size_t n = 0; n != size; ++n) {
+        if (!unit) {
+            if (*at(A, n, n) == 0.0) { g_trsv_singular = 1; return; }
+            b[(ptrdiff_t)n * inc] /= *at(A, n, n);
+        }
+        const double bn = b[(ptrdiff_t)n * inc];
+        if (bn != 0.0) {
+            const double s = -bn;
+            for (size_t i = n + 1; i < size; ++i) b[(ptrdiff_t)i * inc] = b[(ptrdiff_t)i * inc] + s * *at(A, i, n);
+        }
+    }
+}
+
+/* lower, row-major storage: b_n -= dot(b[0:n], A[n, 0:n]), then / A_nn
+ * (default/trsv.hpp:70-96). */
+static void trsv_lower_rm(view A, double *b, ptrdiff_t inc, int unit) {
+    const size_t size = A.m;
+    for (size_t n = 0; n < size; ++n) {
+#ifdef ORC_MUTANT_TRSV_SKIP_FIRST
+        double value = n > 0 ? dot_strided(b + inc, inc, at(A, n, 1), A.s2, n - 1) : 0.0; /* planted: drops k = 0 */
+#else
+        double value = dot_strided(b, inc, at(A, n, 0), A.s2, n);
+#endif
+        b[(ptrdiff_t)n * inc] -= value;
+        if (!unit) {
+            if (*at(A, n, n) == 0.0) { g_trsv_singular = 1; return; }
+            b[(ptrdiff_t)n * inc] /= *at(A, n, n);
+        }
+    }
+}
+
+/* upper, column-major storage: back to front, b_n /= A_nn, then
+ * b[0:n] = b[0:n] + (-b_n) * A[0:n, n], skipped when b_n == 0
+ * (default/trsv.hpp:98-124). */
+static void trsv_upper_cm(view A, double *b, ptrdiff_t inc, int unit) {
+    const size_t size = A.m;
+    for (size_t i = 0; i < size; ++i) {
+        const size_t n = size - i - 1;
+        if (!unit) {
+            if (*at(A, n, n) == 0.0) { g_trsv_singular = 1; return; }
+            b[(ptrdiff_t)n * inc] /= *at(A, n, n);
+        }
+        const double bn = b[(ptrdiff_t)n * inc];
+        if (bn != 0.0) {
+            const double s = -bn;
+            for (size_t r = 0; r < n; ++r) b[(ptrdiff_t)r * inc] = b[(ptrdiff_t)r * inc] + s * *at(A, r, n);
+        }
+    }
+}
+
+/* upper, row-major storage: back to front, b_n -= dot(b[n+1:], A[n, n+1:]),
+ * then / A_nn (default/trsv.hpp:125-151). */
+static void trsv_upper_rm(view A, double *b, ptrdiff_t inc, int unit) {
+    const size_t size = A.m;
+    for (size_t i = 0; i < size; ++i) {
+        const size_t n = size - i - 1;
+        double value = dot_strided(b + (ptrdiff_t)(n + 1) * inc, inc, at(A, n, n + 1), A.s2, size - n - 1);
+        b[(ptrdiff_t)n * inc] -= value;
+        if (!unit) {
+            if (*at(A, n, n) == 0.0) { g_trsv_singular = 1; return; }
+            b[(ptrdiff_t)n * inc] /= *at(A, n, n);
+        }
+    }
+}
+
+/* trsv<Triangular, Side>(A, b) (kernels/trsv.hpp:53-63 -> default/trsv.hpp:154-183):
+ * dispatch on the orientation tag; the right side solves b A = b as
+ * A^T x = b through trans(A), which flips the triangle and the tag. */
+static void kernels_trsv(view A, double *b, ptrdiff_t inc, int upper, int unit, int left) {
+    if (!left) {
+        A = trans(A);
+        upper = !upper;
+    }
+    if (upper) {
+        if (A.row_major) trsv_upper_rm(A, b, inc, unit);
+        else trsv_upper_cm(A, b, inc, unit);
+    } else {
+        if (A.row_major) trsv_lower_rm(A, b, inc, unit);
+        else trsv_lower_cm(A, b, inc, unit);
+    }
+}
+
+/* ============================================================ gemv.hpp == */
+/* result += alpha * A * x (kernels/gemv.hpp:58-72 -> default/gemv.hpp:43-93):
+ *   row-major A:    value = dot(row(A, i), x); if value != 0,
+ *                   result_i += alpha * value;
+ *   column-major A: for each j, result = result + (alpha * x_j) * column(A, j). */
+static void kernels_gemv(view A, const double *x, ptrdiff_t incx, double *y, ptrdiff_t incy, double alpha) {
+    if (A.row_major) {
+        for (size_t i = 0; i != A.m; ++i) {
+            double value = dot_strided(at(A, i, 0), A.s2, x, incx, A.n);
+            if (value != 0.0) y[(ptrdiff_t)i * incy] += alpha * value;
+        }
+    } else {
+        for (size_t j = 0; j != A.n; ++j) {
+#ifdef ORC_MUTANT_GEMV_NO_ALPHA
+            const double s = x[(ptrdiff_t)j * incx]; /* planted: alpha dropped */
+#else
+            const double s = alpha * x[(ptrdiff_t)j * incx];
+#endif
+            for (size_t i = 0; i != A.m; ++i) y[(ptrdiff_t)i * incy] = y[(ptrdiff_t)i * incy] + s * *at(A, i, j);
+        }
+    }
+}
+
+/* Solve op-free A x = b (left) or x A = b (right) in place; A n x n
+ * triangular.  Returns 1 where the reference throws "[TRSV] Matrix is
+ * singular!" (a zero diagonal, non-unit only); b then holds the partial
+ * result. */
+int orc_dtrsv(const double *A, size_t n, size_t lda, int a_row_major, double *b, ptrdiff_t incb, int upper,
+              int unit, int left) {
+    g_trsv_singular = 0;
+    if (n == 0) return 0;
+    kernels_trsv(make_view((double *)A, n, n, lda, a_row_major), b, incb, upper, unit, left);
+    return g_trsv_singular;
+}
+
+/* y(m) += alpha * A(m x n) * x(n) */
+void orc_dgemv(const double *A, size_t m, size_t n, size_t lda, int a_row_major, const double *x, ptrdiff_t incx,
+               double *y, ptrdiff_t incy, double alpha) {
+    if (m == 0 || n == 0) return;
+    kernels_gemv(make_view((double *)A, m, n, lda, a_row_major), x, incx, y, incy, alpha);
+}
+
+/* cholesky_decomposition(A) followed by solve(b) for a vector
+ * (decompositions.hpp:81-91, 111-115): potrf<lower>, then
+ * trsv<lower,left>(L) and trsv<upper,left>(trans(L)), for either side (A is
+ * symmetric).  A is overwritten by its factor, b by the solution. */
+int orc_dposv_vec(double *A, size_t n, size_t lda, int a_row_major, double *b, ptrdiff_t incb, size_t *info) {
+    view L = make_view(A, n, n, lda, a_row_major);
+    *info = kernels_potrf(L, 0);
+    g_trsv_singular = 0;
+    if (n == 0) return 0;
+    kernels_trsv(L, b, incb, 0, 0, 1);
+    if (!g_trsv_singular) kernels_trsv(trans(L), b, incb, 1, 0, 1);
+    return g_trsv_singular;
 }
 
 /* The build's SIMD width, for tests that pin the block constants. */

@@ -72,3 +72,18 @@ def prod_operands(rows=80, middle=33, columns=50):
     j2 = np.arange(columns)[None, :]
     B = i2 * columns + 1.5 * j2
     return A.astype(np.float64), B.astype(np.float64)
+
+
+def triangular_vector_rhs(size: int = 158) -> np.ndarray:
+    """b(i) = 0.1/size*i (Test/triangular_solve.cpp:31-34)."""
+    return (0.1 / size) * np.arange(size, dtype=np.float64)
+
+
+def prod_vector_operands(rows: int = 50, columns: int = 40):
+    """arg1(i,j) = 0.01*i*columns + 0.1*j, arg2(j) = 0.1*j + 0.1
+    (Test/hip_prod.cpp:33-49); the cases compute result(1.5) += -2 * arg1 arg2."""
+    i = np.arange(rows)[:, None]
+    j = np.arange(columns)[None, :]
+    A = 0.01 * i * columns + 0.1 * j
+    x = 0.1 * np.arange(columns) + 0.1
+    return A.astype(np.float64), x.astype(np.float64)

@@ -370,6 +370,144 @@ def test_pstrf_ties_take_the_first_maximum():
     assert P[0] == 1 and r == 4
 
 
+# ----------------------------------------------------------------- trsv --
+TRSV_CASES = [(left, upper, unit) for left in (True, False) for upper in (False, True) for unit in (False, True)]
+
+
+def _tri(A, upper, unit):
+    T = np.triu(A) if upper else np.tril(A)
+    if unit:
+        T = T - np.diag(np.diag(T)) + np.eye(len(A))
+    return T
+
+
+@pytest.mark.parametrize("left,upper,unit", TRSV_CASES)
+@pytest.mark.parametrize("a_row_major", ORIENT)
+def test_trsv_reference_fixture(left, upper, unit, a_row_major):
+    """Test/triangular_solve.cpp:19-110 (size 158): A x = b (left) or
+    x A = b (right) to 1e-11, and LAPACK's trtrs (scipy) to rounding."""
+    Alow = fixtures.triangular_matrix(158, False)
+    A = Alow if not upper else Alow.T.copy()
+    b = fixtures.triangular_vector_rhs(158)
+    x, sing = orc.trsv(A, b, upper, unit, left, a_row_major=a_row_major)
+    assert not sing
+    T = _tri(A, upper, unit)
+    r = T @ x - b if left else x @ T - b
+    assert np.max(np.abs(r)) < 1e-11
+    ref = sla.solve_triangular(T if left else T.T, b, lower=(not upper) if left else upper)
+    np.testing.assert_allclose(x, ref, rtol=1e-13, atol=1e-15 * np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("left,upper,unit", TRSV_CASES)
+@pytest.mark.parametrize("a_row_major", ORIENT)
+def test_trsv_vs_mpmath(left, upper, unit, a_row_major):
+    """40-digit solves of random triangular systems (n = 29): every variant
+    of the reference's trsv_impl agrees to 1e-14 of the solution's size."""
+    import mpmath
+    mpmath.mp.dps = 40
+    n = 29
+    A = synth.uniform(3, 0, n * n, -1, 1).reshape(n, n) + 3 * np.eye(n)
+    b = synth.uniform(3, 1, n, -1, 1)
+    x, _ = orc.trsv(A, b, upper, unit, left, a_row_major=a_row_major)
+    T = _tri(A, upper, unit)
+    M = mpmath.matrix((T if left else T.T).tolist())
+    xe = np.array([float(v) for v in mpmath.lu_solve(M, mpmath.matrix(b.tolist()))])
+    assert np.max(np.abs(x - xe)) < 1e-14 * np.max(np.abs(xe))
+
+
+@pytest.mark.parametrize("a_row_major", ORIENT)
+@pytest.mark.parametrize("inc", [1, 3])
+def test_trsv_vs_lapack_strided(a_row_major, inc):
+    n = 400
+    A = np.tril(synth.uniform(5, 0, n * n, -1, 1).reshape(n, n)) / np.sqrt(n) + 2 * np.eye(n)
+    b = synth.uniform(5, 1, n, -1, 1)
+    for upper, T in ((False, A), (True, A.T.copy())):
+        x, _ = orc.trsv(T, b, upper, False, True, a_row_major=a_row_major, inc=inc)
+        ref = sla.solve_triangular(T, b, lower=not upper)
+        np.testing.assert_allclose(x, ref, rtol=0, atol=1e-13 * np.max(np.abs(ref)))
+
+
+def test_trsv_exact_on_integers():
+    """A unit lower triangle of small integers and an integer b: the solution
+    is integral and every intermediate exact, so both storage orders (the
+    column sweep and the row dot products) must give the exact answer."""
+    from fractions import Fraction
+    n = 14
+    rng = np.random.default_rng(2)
+    A = np.tril(rng.integers(-1, 2, (n, n)).astype(float), -1) + np.diag(rng.integers(2, 9, n).astype(float))
+    b = rng.integers(-5, 6, n).astype(float)
+    x_exact = []
+    for i in range(n):  # unit solve in exact rationals
+        x_exact.append(Fraction(int(b[i])) - sum(Fraction(int(A[i, k])) * x_exact[k] for k in range(i)))
+    xe = np.array([float(v) for v in x_exact])
+    for rm in ORIENT:
+        x, _ = orc.trsv(A, b, False, True, True, a_row_major=rm)
+        assert np.array_equal(x, xe)
+
+
+def test_trsv_special_cases():
+    n = 33
+    b = synth.uniform(6, 1, n, -1, 1)
+    for rm in ORIENT:
+        x, _ = orc.trsv(np.eye(n), b, False, False, True, a_row_major=rm)
+        assert np.array_equal(x, b)
+        d = synth.uniform(6, 2, n, 1, 2)
+        x, _ = orc.trsv(np.diag(d), b, True, False, True, a_row_major=rm)
+        assert np.array_equal(x, b / d)
+        # unit: the stored diagonal is never read
+        G = np.tril(synth.uniform(6, 3, n * n, -1, 1).reshape(n, n)) / n
+        Gd = G.copy()
+        Gd[np.diag_indices(n)] = 7.0
+        x1, _ = orc.trsv(G, b, False, True, True, a_row_major=rm)
+        x2, _ = orc.trsv(Gd, b, False, True, True, a_row_major=rm)
+        assert np.array_equal(x1, x2)
+        # a zero diagonal: the reference throws "[TRSV] Matrix is singular!"
+        Z = np.tril(np.ones((n, n)))
+        Z[10, 10] = 0.0
+        assert orc.trsv(Z, b, False, False, True, a_row_major=rm)[1]
+        assert not orc.trsv(Z, b, False, True, True, a_row_major=rm)[1]
+
+
+# ----------------------------------------------------------------- gemv --
+@pytest.mark.parametrize("a_row_major", ORIENT)
+def test_gemv_reference_fixture(a_row_major):
+    """Test/hip_prod.cpp:33-106: result(1.5) += -2 A x for A 50 x 40 (and
+    A^T x through the other orientation), against the textbook loop."""
+    A, x = fixtures.prod_vector_operands()
+    y = orc.gemv(A, x, np.full(50, 1.5), -2.0, a_row_major=a_row_major)
+    ref = np.array([1.5 + sum(-2.0 * A[i, k] * x[k] for k in range(40)) for i in range(50)])
+    np.testing.assert_allclose(y, ref, rtol=1e-14)
+    xt = 0.1 * np.arange(50) + 0.1
+    yt = orc.gemv(A.T.copy(), xt, np.zeros(40), 1.0, a_row_major=a_row_major)
+    np.testing.assert_allclose(yt, A.T @ xt, rtol=1e-14)
+
+
+def test_gemv_exact_on_integers():
+    """Small integers, alpha = -2 and y = 1.5: every product and partial sum
+    is exact, so both orientations (row dots, column axpys) equal numpy."""
+    rng = np.random.default_rng(4)
+    A = rng.integers(-9, 10, (70, 45)).astype(float)
+    x = rng.integers(-9, 10, 45).astype(float)
+    ref = 1.5 - 2.0 * (A @ x)
+    for rm in ORIENT:
+        assert np.array_equal(orc.gemv(A, x, np.full(70, 1.5), -2.0, a_row_major=rm), ref)
+
+
+# ------------------------------------------------------ vector SPD solve --
+@pytest.mark.parametrize("a_row_major", ORIENT)
+def test_posv_vector_reference_fixture(a_row_major):
+    """Test/symm_solve.cpp:89-139 restated for one right-hand side: A x = b
+    to 1e-12; the vector path (trsv) agrees with the matrix path (trsm) on
+    the same column."""
+    A = fixtures.create_symm(128)
+    b = fixtures.symm_solve_rhs(128, 151)[:, 17].copy()
+    x, _, info, sing = orc.posv_vec(A, b, a_row_major=a_row_major)
+    assert info == 0 and not sing
+    assert np.max(np.abs(A @ x - b)) < 1e-12
+    X, _, _, _ = orc.posv(A, b[:, None], a_row_major=a_row_major)
+    np.testing.assert_allclose(x, X[:, 0], rtol=1e-13, atol=1e-15)
+
+
 # -------------------------------------------------------------- mutants --
 @pytest.fixture(scope="module")
 def mutant_dir(tmp_path_factory):
@@ -430,3 +568,18 @@ def test_mutant_pstrf_last_max_is_caught(mutant_dir):
     lib = _mutant(mutant_dir, "PSTRF_LAST_MAX")
     F, P, r = orc.pstrf(np.eye(5), lib=lib)
     assert not np.array_equal(P, np.arange(5))
+
+
+def test_mutant_trsv_skip_first_is_caught(mutant_dir):
+    lib = _mutant(mutant_dir, "TRSV_SKIP_FIRST")
+    A = fixtures.triangular_matrix(158, False)
+    b = fixtures.triangular_vector_rhs(158) + 0.5  # the fixture's b(0) = 0 would hide a dropped first term
+    x, _ = orc.trsv(A, b, False, False, True, a_row_major=True, lib=lib)
+    assert np.max(np.abs(np.tril(A) @ x - b)) > 1e-11
+
+
+def test_mutant_gemv_no_alpha_is_caught(mutant_dir):
+    lib = _mutant(mutant_dir, "GEMV_NO_ALPHA")
+    A, x = fixtures.prod_vector_operands()
+    y = orc.gemv(A, x, np.full(50, 1.5), -2.0, a_row_major=False, lib=lib)
+    assert not np.allclose(y, 1.5 - 2.0 * A @ x, rtol=1e-12)
