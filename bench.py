@@ -442,6 +442,7 @@ def main():
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-rbf", action="store_true", help="skip the RBF legs (FP64 and mixed); profiling runs only")
     ap.add_argument("--no-gram", action="store_true", help="skip the FP64 Gram-matrix leg")
     ap.add_argument("--only-fp64", action="store_true", help="FP64 legs only (no mixed leg)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-systems legs")
@@ -472,48 +473,49 @@ def main():
         la.set_potrf_lookahead(h, args.lookahead_mode)
     paths, acc = {}, {}
 
-    log(f"[bench] rbf inputs n={n} nrhs={nrhs}")
-    A, B = make_rbf(n, nrhs)
-    torch.cuda.synchronize()
-    log("[bench] rbf fp64 potrf+potrs")
-    r = run_fp64(h, A, B, args.steps, args.warmup)
-    med64 = median(r["step_ms"])
-    eta = backward_error(h, A, r["X"], B)
-    ferr = factor_error_sampled(A, r["L"])
-    paths["rbf_fp64"] = {"tflops": work / (med64 * 1e-3) / 1e12, "ms": med64, "step_ms": r["step_ms"]}
-    acc["rbf_fp64"] = {"backward_error": eta, "factor_error_sampled": ferr}
-    roof64 = roofline(r["trailing"], "f64", "gemm_kernel<double,128,128,16,false,true,1,0>: trailing SYRK update, "
-                                                  "lower trapezoid, K = panel width (512 for FP64)")
-    log(f"[bench] rbf fp64: median {med64:.3f} ms  {paths['rbf_fp64']['tflops']:.2f} TFLOP/s  eta {eta:.2e}  "
-        f"ferr {ferr:.2e}")
-    del r
-    roof32 = None
-    if not args.only_fp64:
-        log("[bench] rbf mixed fp32 dsposv")
-        r = run_mixed(h, A, B, args.steps, args.warmup, "fp32")
-        medmx = median(r["step_ms"])
+    roof64 = roof32 = None
+    if not args.no_rbf:
+        log(f"[bench] rbf inputs n={n} nrhs={nrhs}")
+        A, B = make_rbf(n, nrhs)
+        torch.cuda.synchronize()
+        log("[bench] rbf fp64 potrf+potrs")
+        r = run_fp64(h, A, B, args.steps, args.warmup)
+        med64 = median(r["step_ms"])
         eta = backward_error(h, A, r["X"], B)
-        paths["rbf_mixed_fp32"] = {"tflops": work / (medmx * 1e-3) / 1e12, "ms": medmx, "step_ms": r["step_ms"],
-                                   "iters": r["iters"]}
-        acc["rbf_mixed_fp32"] = {"backward_error": eta, "reported_berr": r["berr"]}
-        roof32 = roofline(r["trailing"], "f32", "gemm_kernel<float,128,128,32,false,true,1,0>: trailing SYRK update, "
-                                                  "lower trapezoid, K = panel width (256 for FP32)")
-        log(f"[bench] rbf mixed: median {medmx:.3f} ms  {paths['rbf_mixed_fp32']['tflops']:.2f} TFLOP/s  "
-            f"eta {eta:.2e}  iters {r['iters']}")
-        assert max(r["iters"]) >= 0 and eta <= TOL, "mixed path did not reach the tolerance"
+        ferr = factor_error_sampled(A, r["L"])
+        paths["rbf_fp64"] = {"tflops": work / (med64 * 1e-3) / 1e12, "ms": med64, "step_ms": r["step_ms"]}
+        acc["rbf_fp64"] = {"backward_error": eta, "factor_error_sampled": ferr}
+        roof64 = roofline(r["trailing"], "f64", "gemm_kernel<double,128,128,16,false,true,1,0>: trailing SYRK update, "
+                                                      "lower trapezoid, K = panel width (512 for FP64)")
+        log(f"[bench] rbf fp64: median {med64:.3f} ms  {paths['rbf_fp64']['tflops']:.2f} TFLOP/s  eta {eta:.2e}  "
+            f"ferr {ferr:.2e}")
         del r
-        log("[bench] rbf mixed split-bf16x3 dsposv")
-        r = run_mixed(h, A, B, args.steps, args.warmup, "bf16x3", profile=False)
-        medb = median(r["step_ms"])
-        eta = backward_error(h, A, r["X"], B)
-        paths["rbf_mixed_bf16x3"] = {"tflops": work / (medb * 1e-3) / 1e12, "ms": medb, "step_ms": r["step_ms"],
-                                     "iters": r["iters"], "fell_back_to_fp64": any(i < 0 for i in r["iters"])}
-        acc["rbf_mixed_bf16x3"] = {"backward_error": eta, "reported_berr": r["berr"]}
-        log(f"[bench] rbf mixed bf16x3: median {medb:.3f} ms  {paths['rbf_mixed_bf16x3']['tflops']:.2f} TFLOP/s  "
-            f"eta {eta:.2e}  iters {r['iters']}")
-        del r
-    del A, B
-    torch.cuda.empty_cache()
+        if not args.only_fp64:
+            log("[bench] rbf mixed fp32 dsposv")
+            r = run_mixed(h, A, B, args.steps, args.warmup, "fp32")
+            medmx = median(r["step_ms"])
+            eta = backward_error(h, A, r["X"], B)
+            paths["rbf_mixed_fp32"] = {"tflops": work / (medmx * 1e-3) / 1e12, "ms": medmx, "step_ms": r["step_ms"],
+                                       "iters": r["iters"]}
+            acc["rbf_mixed_fp32"] = {"backward_error": eta, "reported_berr": r["berr"]}
+            roof32 = roofline(r["trailing"], "f32", "gemm_kernel<float,128,128,32,false,true,1,0>: trailing SYRK update, "
+                                                      "lower trapezoid, K = panel width (256 for FP32)")
+            log(f"[bench] rbf mixed: median {medmx:.3f} ms  {paths['rbf_mixed_fp32']['tflops']:.2f} TFLOP/s  "
+                f"eta {eta:.2e}  iters {r['iters']}")
+            assert max(r["iters"]) >= 0 and eta <= TOL, "mixed path did not reach the tolerance"
+            del r
+            log("[bench] rbf mixed split-bf16x3 dsposv")
+            r = run_mixed(h, A, B, args.steps, args.warmup, "bf16x3", profile=False)
+            medb = median(r["step_ms"])
+            eta = backward_error(h, A, r["X"], B)
+            paths["rbf_mixed_bf16x3"] = {"tflops": work / (medb * 1e-3) / 1e12, "ms": medb, "step_ms": r["step_ms"],
+                                         "iters": r["iters"], "fell_back_to_fp64": any(i < 0 for i in r["iters"])}
+            acc["rbf_mixed_bf16x3"] = {"backward_error": eta, "reported_berr": r["berr"]}
+            log(f"[bench] rbf mixed bf16x3: median {medb:.3f} ms  {paths['rbf_mixed_bf16x3']['tflops']:.2f} TFLOP/s  "
+                f"eta {eta:.2e}  iters {r['iters']}")
+            del r
+        del A, B
+        torch.cuda.empty_cache()
 
     if not args.no_gram:
         log(f"[bench] gram inputs n={n} nrhs={nrhs}")
@@ -571,6 +573,9 @@ def main():
         log(f"[bench] cpu: {cpu['value'] * 1e3:.2f} GFLOP/s")
 
     mixed = "rbf_mixed_fp32" in paths
+    if not mixed and "rbf_fp64" not in paths:  # a profiling run without the headline legs
+        print(json.dumps({"paths": paths}), flush=True)
+        return
     head = paths["rbf_mixed_fp32"] if mixed else paths["rbf_fp64"]
     out = {
         "metric": "potrf+solve TFLOP/s n=16384: FP64 vs FP32-MFMA+refinement @ equal residual",

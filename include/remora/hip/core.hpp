@@ -15,6 +15,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstddef>
+#include <cstdint>
 #include <memory>
 #include <string>
 #include <system_error>
@@ -184,6 +185,32 @@ inline rmh_handle get_handle(device const& d) {
     rmh_handle h = nullptr;
     check_rmh(rmh_create(&h, d.device_id()));
     handles->emplace_back(d.device_id(), h);
+    return h;
+}
+
+// A few device int64 words per thread and device for the status values the
+// front end reads back (potrf's info, pstrf's rank, update's failing
+// column): allocated on first use and kept until the process ends, so those
+// calls allocate nothing after the first.  Calls on one thread use its
+// stream in order and read the word back before returning, so one word per
+// kind is enough.
+enum status_word { status_info = 0, status_rank = 1, status_update = 2, status_words_count = 4 };
+inline int64_t* status_words(device const& d, status_word w = status_info) {
+    thread_local std::vector<std::pair<int, int64_t*>>* words = new std::vector<std::pair<int, int64_t*>>();
+    for (auto& e : *words)
+        if (e.first == d.device_id()) return e.second + w;
+    d.set_device();
+    int64_t* p = nullptr;
+    check_hip(hipMalloc(&p, status_words_count * sizeof(int64_t)), "hipMalloc");
+    words->emplace_back(d.device_id(), p);
+    return p + w;
+}
+// Wait for the thread's stream and return the word's value.
+inline int64_t read_status_word(device const& d, const int64_t* word) {
+    stream& s = get_stream(d);
+    int64_t h = 0;
+    check_hip(hipMemcpyAsync(&h, word, sizeof(h), hipMemcpyDeviceToHost, s.handle()), "hipMemcpyAsync");
+    s.synchronize();
     return h;
 }
 

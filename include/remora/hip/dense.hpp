@@ -16,9 +16,11 @@
 #include <cassert>
 #include <cstring>
 #include <mutex>
+#include <stdexcept>
 #include <type_traits>
 #include <vector>
 
+#include "abi.hpp"
 #include "core.hpp"
 
 namespace remora {
@@ -38,6 +40,19 @@ template <class E, class Device>
 struct vector_expression {
     typedef Device device_type;
     E& operator()() { return *static_cast<E*>(this); }
+    E const& operator()() const { return *static_cast<E const*>(this); }
+};
+
+// Expressions evaluated into a target by their own kernels (products,
+// solves): they give size, queue and assign_to / plus_assign_to, the way the
+// reference's blockwise expressions do (detail/matrix_expression_classes.hpp,
+// solve.hpp).
+template <class E>
+struct matrix_lazy_expression {
+    E const& operator()() const { return *static_cast<E const*>(this); }
+};
+template <class E>
+struct vector_lazy_expression {
     E const& operator()() const { return *static_cast<E const*>(this); }
 };
 
@@ -84,6 +99,32 @@ private:
     hip::device* m_queue;
 };
 
+namespace detail {
+// dst = src for dense device matrices of the same size, any orientations, on
+// the device: the same layout is a 2-D copy of the lines (rows for
+// row_major, columns otherwise), a different layout a transposed copy
+// (rmh_?copy_matrix); only the size1 x size2 elements of src are read and of
+// dst written.
+template <class Dst, class Src>
+void copy_dense(Dst const& dst_m, Src const& src_m) {
+    typedef typename Dst::orientation O;
+    if (dst_m.size1() != src_m.size1() || dst_m.size2() != src_m.size2())
+        throw std::invalid_argument("[assign] sizes do not match");
+    if (dst_m.size1() == 0 || dst_m.size2() == 0) return;
+    auto src = src_m.raw_storage();
+    auto dst = dst_m.raw_storage();
+    hip::device& q = dst_m.queue();
+    q.set_device();
+    // the destination's memory as a column-major matrix
+    const std::size_t rows = O::is_row_major ? dst_m.size2() : dst_m.size1();
+    const std::size_t cols = O::is_row_major ? dst_m.size1() : dst_m.size2();
+    const bool transposed = !std::is_same<typename Src::orientation, O>::value;
+    hip::check_rmh(hip::abi::copy_matrix(hip::get_handle(q), transposed, rows, cols, src.values,
+                                         src.leading_dimension, dst.values, dst.leading_dimension,
+                                         hip::get_stream(q).abi()));
+}
+}  // namespace detail
+
 // ---------------------------------------------------- device container --
 template <class T, class Orientation = row_major, class Device = cpu_tag>
 class matrix;
@@ -118,6 +159,17 @@ public:
         assign_from(e());
         return *this;
     }
+    template <class E>
+    matrix(matrix_lazy_expression<E> const& e) : matrix(e().size1(), e().size2(), e().queue()) {
+        e().assign_to(*this);
+    }
+    template <class E>
+    matrix& operator=(matrix_lazy_expression<E> const& e) {
+        // a product or solve may read this matrix: evaluate into a temporary
+        matrix tmp(e);
+        *this = std::move(tmp);
+        return *this;
+    }
 
     std::size_t size1() const { return m_size1; }
     std::size_t size2() const { return m_size2; }
@@ -131,37 +183,9 @@ public:
     }
 
 private:
-    // element-wise copy of any dense device view, orientation-aware
     template <class E>
     void assign_from(E const& e) {
-        auto src = e.raw_storage();
-        auto dst = raw_storage();
-        hip::stream& s = hip::get_stream(*m_queue);
-        m_queue->set_device();
-        if (m_size1 == 0 || m_size2 == 0) return;
-        if (std::is_same<typename E::orientation, Orientation>::value) {
-            // same layout: 2-D copy of "lines" (rows for row_major, columns otherwise)
-            const std::size_t lines = Orientation::is_row_major ? m_size1 : m_size2;
-            const std::size_t len = Orientation::is_row_major ? m_size2 : m_size1;
-            hip::check_hip(hipMemcpy2DAsync(dst.values, dst.leading_dimension * sizeof(T), src.values,
-                                            src.leading_dimension * sizeof(T), len * sizeof(T), lines,
-                                            hipMemcpyDeviceToDevice, s.handle()),
-                           "hipMemcpy2DAsync");
-        } else {
-            // transposed layouts: bounce through the host (tests / setup only)
-            std::vector<T> h(src.leading_dimension * (E::orientation::is_row_major ? e.size1() : e.size2()));
-            hip::check_hip(hipMemcpyAsync(h.data(), src.values, h.size() * sizeof(T), hipMemcpyDeviceToHost,
-                                          s.handle()), "hipMemcpyAsync");
-            s.synchronize();
-            std::vector<T> out(m_size1 * m_size2);
-            for (std::size_t i = 0; i < m_size1; ++i)
-                for (std::size_t j = 0; j < m_size2; ++j)
-                    out[detail::element_index<Orientation>(i, j, dst.leading_dimension)] =
-                        h[detail::element_index<typename E::orientation>(i, j, src.leading_dimension)];
-            hip::check_hip(hipMemcpyAsync(dst.values, out.data(), out.size() * sizeof(T), hipMemcpyHostToDevice,
-                                          s.handle()), "hipMemcpyAsync");
-            s.synchronize();
-        }
+        detail::copy_dense(*this, e);
     }
 
     hip::device* m_queue;
@@ -169,7 +193,42 @@ private:
     hip::buffer<T> m_data;
 };
 
-// device vector (contiguous); vector views carry a stride
+// Non-owning view of a dense device vector with a stride (the role of
+// dense_vector_adaptor<T, hip_tag>).
+template <class T, class Device>
+class vector_view;
+
+template <class T>
+class vector_view<T, hip_tag> : public vector_expression<vector_view<T, hip_tag>, hip_tag> {
+public:
+    typedef T value_type;
+    typedef dense_vector_storage<T> storage_type;
+    vector_view(T* values, std::size_t size, std::size_t stride, hip::device& q)
+        : m_values(values), m_size(size), m_stride(stride), m_queue(&q) {}
+    std::size_t size() const { return m_size; }
+    storage_type raw_storage() const { return {m_values, m_stride}; }
+    hip::device& queue() const { return *m_queue; }
+
+private:
+    T* m_values;
+    std::size_t m_size, m_stride;
+    hip::device* m_queue;
+};
+
+namespace detail {
+// dst[i * sd] = src[i * ss] for i < n on the thread's stream (a 2-D copy of
+// n one-element rows, so any strides work)
+template <class T>
+void copy_vector(T* dst, std::size_t sd, const T* src, std::size_t ss, std::size_t n, hip::device& q) {
+    if (n == 0) return;
+    q.set_device();
+    hip::check_hip(hipMemcpy2DAsync(dst, sd * sizeof(T), src, ss * sizeof(T), sizeof(T), n, hipMemcpyDeviceToDevice,
+                                    hip::get_stream(q).handle()),
+                   "hipMemcpy2DAsync");
+}
+}  // namespace detail
+
+// device vector (contiguous)
 template <class T, class Device = cpu_tag>
 class vector;
 
@@ -178,11 +237,56 @@ class vector<T, hip_tag> : public vector_expression<vector<T, hip_tag>, hip_tag>
 public:
     typedef T value_type;
     typedef dense_vector_storage<T> storage_type;
+    typedef hip_tag device_type;
     explicit vector(std::size_t size = 0, hip::device& q = hip::device_manager::instance().default_device())
         : m_queue(&q), m_data(size, q) {}
+    // every element set to init (the reference's vector(size, value))
+    vector(std::size_t size, T init, hip::device& q = hip::device_manager::instance().default_device())
+        : m_queue(&q), m_data(size, q) {
+        if (size == 0) return;
+        std::vector<T> h(size, init);
+        q.set_device();
+        hip::stream& s = hip::get_stream(q);
+        hip::check_hip(hipMemcpyAsync(m_data.get(), h.data(), size * sizeof(T), hipMemcpyHostToDevice, s.handle()),
+                       "hipMemcpyAsync");
+        s.synchronize();
+    }
+    vector(vector const& o) : vector(o.size(), o.queue()) { detail::copy_vector(m_data.get(), 1, o.m_data.get(), 1, size(), *m_queue); }
+    vector(vector&&) noexcept = default;
+    template <class E>
+    vector(vector_expression<E, hip_tag> const& e) : vector(e().size(), e().queue()) {
+        auto src = e().raw_storage();
+        detail::copy_vector(m_data.get(), 1, src.values, src.stride, size(), *m_queue);
+    }
+    template <class E>
+    vector(vector_lazy_expression<E> const& e) : vector(e().size(), e().queue()) {
+        e().assign_to(*this);
+    }
+    vector& operator=(vector const& o) {
+        resize(o.size());
+        detail::copy_vector(m_data.get(), 1, o.m_data.get(), 1, size(), *m_queue);
+        return *this;
+    }
+    vector& operator=(vector&&) noexcept = default;
+    template <class E>
+    vector& operator=(vector_expression<E, hip_tag> const& e) {
+        auto src = e().raw_storage();
+        resize(e().size());
+        detail::copy_vector(m_data.get(), 1, src.values, src.stride, size(), *m_queue);
+        return *this;
+    }
+    template <class E>
+    vector& operator=(vector_lazy_expression<E> const& e) {
+        vector tmp(e);  // the expression may read this vector
+        *this = std::move(tmp);
+        return *this;
+    }
     std::size_t size() const { return m_data.size(); }
     storage_type raw_storage() const { return {m_data.get(), 1}; }
     hip::device& queue() const { return *m_queue; }
+    void resize(std::size_t n) {
+        if (n != m_data.size()) m_data = hip::buffer<T>(n, *m_queue);
+    }
 
 private:
     hip::device* m_queue;
@@ -230,6 +334,22 @@ public:
 
 private:
     std::size_t m_size1 = 0, m_size2 = 0;
+    std::vector<T> m_data;
+};
+
+// Minimal host vector used by the transport and the tests.
+template <class T>
+class vector<T, cpu_tag> {
+public:
+    typedef T value_type;
+    explicit vector(std::size_t n = 0, T init = T()) : m_data(n, init) {}
+    std::size_t size() const { return m_data.size(); }
+    T& operator()(std::size_t i) { return m_data[i]; }
+    T const& operator()(std::size_t i) const { return m_data[i]; }
+    T* data() { return m_data.data(); }
+    T const* data() const { return m_data.data(); }
+
+private:
     std::vector<T> m_data;
 };
 
@@ -320,6 +440,33 @@ matrix<T, O, cpu_tag> copy_to_cpu(matrix<T, O, hip_tag> const& d) {
     d.queue().set_device();
     hip::detail::download(m.data(), d.raw_storage().values, m.size1() * m.size2() * sizeof(T), s);
     return m;
+}
+
+template <class T>
+vector<T, hip_tag> copy_to_device(vector<T, cpu_tag> const& v, hip_tag,
+                                  hip::device& q = hip::device_manager::instance().default_device()) {
+    vector<T, hip_tag> d(v.size(), q);
+    if (v.size() == 0) return d;
+    hip::stream& s = hip::get_stream(q);
+    q.set_device();
+    hip::check_hip(hipMemcpyAsync(d.raw_storage().values, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice,
+                                  s.handle()), "hipMemcpyAsync");
+    s.synchronize();
+    return d;
+}
+
+template <class E>
+vector<typename E::value_type, cpu_tag> copy_to_cpu(vector_expression<E, hip_tag> const& d) {
+    typedef typename E::value_type T;
+    vector<T, cpu_tag> v(d().size());
+    if (v.size() == 0) return v;
+    hip::device& q = d().queue();
+    auto sd = d().raw_storage();
+    vector<T, hip_tag> packed(v.size(), q);  // gathers a strided view
+    detail::copy_vector(packed.raw_storage().values, 1, sd.values, sd.stride, v.size(), q);
+    hip::stream& s = hip::get_stream(q);
+    hip::detail::download(v.data(), packed.raw_storage().values, v.size() * sizeof(T), s);
+    return v;
 }
 
 }  // namespace remora

@@ -10,13 +10,15 @@
 //   solver_traits<MatA, symm_pos_def> ... include/remora/decompositions.hpp:602-609
 //   solve(A, B, symm_pos_def(), left|right) and the lazy
 //   matrix_matrix_solve::assign_to ..... include/remora/solve.hpp:110-182, 459-482
+//   solve(A, b, symm_pos_def(), left|right) for a vector and the lazy
+//   matrix_vector_solve::assign_to ..... include/remora/solve.hpp:40-107, 459-469
 //   inv(A, symm_pos_def()) and prod(inv(A), B) -> solve ... include/remora/solve.hpp:184-248, 432-452, 485-493
 #ifndef REMORA_HIP_SOLVE_HPP
 #define REMORA_HIP_SOLVE_HPP
 
 #include <stdexcept>
 
-#include "kernels.hpp"
+#include "prod.hpp"
 
 namespace remora {
 
@@ -68,14 +70,11 @@ public:
         // row-major storage is the transposed column-major matrix: its lower
         // factor is stored as an upper one
         const bool up = MatrixStorage::orientation::is_row_major;
-        hip::buffer<int64_t> info(1, q);
-        hip::stream& s = hip::get_stream(q);
+        int64_t* info = hip::status_words(q, hip::status_update);
         hip::check_rmh(hip::abi::chol_update(hip::get_handle(q), up, m_cholesky.size1(), alpha, beta, sl.values,
-                                             sl.leading_dimension, sv.values, sv.stride, info.get(), s.abi()));
-        int64_t h = 0;
-        hip::check_hip(hipMemcpyAsync(&h, info.get(), sizeof(h), hipMemcpyDeviceToHost, s.handle()), "hipMemcpyAsync");
-        s.synchronize();
-        if (h != 0)
+                                             sl.leading_dimension, sv.values, sv.stride, info,
+                                             hip::get_stream(q).abi()));
+        if (hip::read_status_word(q, info) != 0)
             throw std::invalid_argument(
                 "[cholesky_decomposition::update] update makes matrix indefinite, no update available");
     }
@@ -111,13 +110,10 @@ public:
         q.set_device();
         hip::stream& s = hip::get_stream(q);
         auto sa = m_factor.raw_storage();
-        hip::buffer<int64_t> rank(1, q);
+        int64_t* rank = hip::status_words(q, hip::status_rank);
         hip::check_rmh(hip::abi::pstrf(hip::get_handle(q), MatrixStorage::orientation::is_row_major, n, sa.values,
-                                       sa.leading_dimension, m_piv.get(), rank.get(), s.abi()));
-        int64_t r = 0;
-        hip::check_hip(hipMemcpyAsync(&r, rank.get(), sizeof(r), hipMemcpyDeviceToHost, s.handle()), "hipMemcpyAsync");
-        s.synchronize();
-        m_rank = std::size_t(r);
+                                       sa.leading_dimension, m_piv.get(), rank, s.abi()));
+        m_rank = std::size_t(hip::read_status_word(q, rank));
         if (m_rank == n || m_rank == 0) return;
         auto L = subrange(m_factor, 0, n, 0, m_rank);
         MatrixStorage LtL(m_rank, m_rank, q);
@@ -202,32 +198,78 @@ struct solver<MatA, symm_semi_pos_def>
 };
 
 // Lazy X = A^-1 B (left) or B A^-1 (right); evaluated on assignment to a
-// device matrix: X <- B, factor a copy of A, solve in place.
+// device matrix: X <- B, factor a copy of A, solve in place
+// (matrix_matrix_solve::assign_to, solve.hpp:165-170).  The operands are
+// held by reference: evaluate the expression within the statement that
+// makes it, as the reference's expression templates require.
 template <class MatA, class MatB, class Side, class Tag = symm_pos_def>
-class matrix_matrix_solve {
+class matrix_matrix_solve : public matrix_lazy_expression<matrix_matrix_solve<MatA, MatB, Side, Tag>> {
 public:
     typedef typename MatB::value_type value_type;
     typedef typename MatB::orientation orientation;
     matrix_matrix_solve(MatA const& A, MatB const& B) : m_A(A), m_B(B) {}
     std::size_t size1() const { return m_B.size1(); }
     std::size_t size2() const { return m_B.size2(); }
+    hip::device& queue() const { return m_B.queue(); }
     template <class MatX>
     void assign_to(MatX& X) const {
-        X = m_B;
+        detail::copy_dense(X, m_B);
         solver<MatA, Tag> alg(m_A);
         alg.solve(X, Side());
     }
-    template <class O>
-    operator matrix<value_type, O, hip_tag>() const {
-        matrix<value_type, O, hip_tag> X(m_B.queue());
-        assign_to(X);
-        return X;
+    template <class MatX>
+    void plus_assign_to(MatX&) const {
+        static_assert(sizeof(MatX) == 0, "[solve] += of a solve is not provided on hip_tag; assign it to a matrix");
+    }
+    matrix_matrix_solve scaled(value_type) const {
+        static_assert(sizeof(MatA) == 0, "[solve] -= of a solve is not provided on hip_tag; assign it to a matrix");
+        return *this;
     }
 
 private:
     MatA const& m_A;
     MatB const& m_B;
 };
+
+// Lazy x = A^-1 b for a vector b (matrix_vector_solve::assign_to,
+// solve.hpp:90-101): x <- b, factor a copy of A, two triangular solves
+// (cholesky_decomposition::solve(b), decompositions.hpp:111-115).
+template <class MatA, class VecB, class Side>
+class matrix_vector_solve : public vector_lazy_expression<matrix_vector_solve<MatA, VecB, Side>> {
+public:
+    typedef typename VecB::value_type value_type;
+    matrix_vector_solve(MatA const& A, VecB const& b) : m_A(A), m_b(b) {}
+    std::size_t size() const { return m_b.size(); }
+    hip::device& queue() const { return m_b.queue(); }
+    template <class VecX>
+    void assign_to(VecX& x) const {
+        if (x.size() != size()) throw std::invalid_argument("[solve] target size does not match");
+        auto sb = m_b.raw_storage();
+        auto sx = x.raw_storage();
+        detail::copy_vector(sx.values, sx.stride, sb.values, sb.stride, size(), x.queue());
+        solver<MatA, symm_pos_def> alg(m_A);
+        alg.solve(x, Side());
+    }
+    template <class VecX>
+    void plus_assign_to(VecX&) const {
+        static_assert(sizeof(VecX) == 0, "[solve] += of a solve is not provided on hip_tag; assign it to a vector");
+    }
+    matrix_vector_solve scaled(value_type) const {
+        static_assert(sizeof(MatA) == 0, "[solve] -= of a solve is not provided on hip_tag; assign it to a vector");
+        return *this;
+    }
+
+private:
+    MatA const& m_A;
+    VecB const& m_b;
+};
+
+template <class MatA, class VecB, class Side>
+matrix_vector_solve<MatA, VecB, Side> solve(matrix_expression<MatA, hip_tag> const& A,
+                                            vector_expression<VecB, hip_tag> const& b, symm_pos_def, Side) {
+    assert(A().size1() == A().size2() && A().size2() == b().size());
+    return matrix_vector_solve<MatA, VecB, Side>(A(), b());
+}
 
 template <class MatA, class MatB, class Side>
 matrix_matrix_solve<MatA, MatB, Side> solve(matrix_expression<MatA, hip_tag> const& A,
@@ -269,6 +311,17 @@ template <class MatB, class MatA, class Tag>
 matrix_matrix_solve<MatA, MatB, right, Tag> prod(matrix_expression<MatB, hip_tag> const& B,
                                                  matrix_inverse<MatA, Tag> const& Ainv) {
     return matrix_matrix_solve<MatA, MatB, right, Tag>(Ainv.A, B());
+}
+// prod(inv(A), b) and prod(b, inv(A)) for a vector b (solve.hpp:432-452)
+template <class MatA, class VecB>
+matrix_vector_solve<MatA, VecB, left> prod(matrix_inverse<MatA, symm_pos_def> const& Ainv,
+                                           vector_expression<VecB, hip_tag> const& b) {
+    return matrix_vector_solve<MatA, VecB, left>(Ainv.A, b());
+}
+template <class VecB, class MatA>
+matrix_vector_solve<MatA, VecB, right> prod(vector_expression<VecB, hip_tag> const& b,
+                                            matrix_inverse<MatA, symm_pos_def> const& Ainv) {
+    return matrix_vector_solve<MatA, VecB, right>(Ainv.A, b());
 }
 
 }  // namespace remora

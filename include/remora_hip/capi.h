@@ -81,6 +81,16 @@ rmh_status rmh_dtrsm(rmh_handle h, int left, int upper, int trans, int unit, siz
 rmh_status rmh_strsm(rmh_handle h, int left, int upper, int trans, int unit, size_t m, size_t n, float alpha,
                      const float* A, size_t lda, float* B, size_t ldb, rmh_stream_t stream);
 
+/* B = alpha*op(A)*B (left) or alpha*B*op(A) (right) in place, A triangular
+ * (upper or lower triangle read, diagonal taken as 1 when unit) of order m
+ * (left) or n (right), B m x n.  Replaces blas_device::trmm
+ * (cublas_backend.hpp:222-263), called by kernels::trmm on hip_tag
+ * (kernels/hip/trmm.hpp:44-80). */
+rmh_status rmh_dtrmm(rmh_handle h, int left, int upper, int trans, int unit, size_t m, size_t n, double alpha,
+                     const double* A, size_t lda, double* B, size_t ldb, rmh_stream_t stream);
+rmh_status rmh_strmm(rmh_handle h, int left, int upper, int trans, int unit, size_t m, size_t n, float alpha,
+                     const float* A, size_t lda, float* B, size_t ldb, rmh_stream_t stream);
+
 /* ------------------------------------------------------------- BLAS-2 -- */
 /* Solve op(A) x = b in place (x has stride incx).  Replaces
  * blas_device::trsv (cublas_backend.hpp:348-383). */
@@ -89,12 +99,31 @@ rmh_status rmh_dtrsv(rmh_handle h, int upper, int trans, int unit, size_t n, con
 rmh_status rmh_strsv(rmh_handle h, int upper, int trans, int unit, size_t n, const float* A, size_t lda,
                      float* x, size_t incx, rmh_stream_t stream);
 
+/* x = op(A) x in place (x has stride incx), A triangular of order n.
+ * Replaces blas_device::trmv (cublas_backend.hpp:265-299), called by
+ * kernels::trmv on hip_tag (kernels/hip/trmv.hpp). */
+rmh_status rmh_dtrmv(rmh_handle h, int upper, int trans, int unit, size_t n, const double* A, size_t lda, double* x,
+                     size_t incx, rmh_stream_t stream);
+rmh_status rmh_strmv(rmh_handle h, int upper, int trans, int unit, size_t n, const float* A, size_t lda, float* x,
+                     size_t incx, rmh_stream_t stream);
+
 /* y = alpha*op(A)*x + beta*y, A is m x n.  Replaces blas_device::gemv
  * (cublas_backend.hpp:170-213). */
 rmh_status rmh_dgemv(rmh_handle h, int trans, size_t m, size_t n, double alpha, const double* A, size_t lda,
                      const double* x, size_t incx, double beta, double* y, size_t incy, rmh_stream_t stream);
 rmh_status rmh_sgemv(rmh_handle h, int trans, size_t m, size_t n, float alpha, const float* A, size_t lda,
                      const float* x, size_t incx, float beta, float* y, size_t incy, rmh_stream_t stream);
+
+/* ------------------------------------------------------------ copies -- */
+/* B = op(A): B is m x n (ld ldb), A is m x n (!trans) or n x m (trans); only
+ * those elements are read and written.  The device side of kernels::assign
+ * between dense hip_tag matrices (include/remora/kernels/hip/matrix_assign.hpp),
+ * which assignments between row- and column-major storage need.  A and B must
+ * not overlap.  Asynchronous. */
+rmh_status rmh_dcopy_matrix(rmh_handle h, int trans, size_t m, size_t n, const double* A, size_t lda, double* B,
+                            size_t ldb, rmh_stream_t stream);
+rmh_status rmh_scopy_matrix(rmh_handle h, int trans, size_t m, size_t n, const float* A, size_t lda, float* B,
+                            size_t ldb, rmh_stream_t stream);
 
 /* ------------------------------------------------------------ LAPACK -- */
 /* In-place Cholesky factorisation A = L L^T (lower) or U^T U (upper); only
@@ -264,6 +293,12 @@ rmh_status rmh_set_potrf_blocking(rmh_handle h, size_t nb, size_t nb_inner);
  * RMH_FACTOR_FP32 (default, FP32 MFMA) or RMH_FACTOR_BF16X3 (split-BF16x3,
  * lower storage).  rmh_dsposv sets its own choice for its factorisation. */
 rmh_status rmh_set_spotrf_precision(rmh_handle h, rmh_factor_precision prec);
+/* Panel kernel of rmh_?potrf for 32-wide diagonal blocks (nb_inner 32, the
+ * default): 1 (the default) factors each panel in one persistent launch,
+ * 0 in one launch per 32-column block column.  Both give the same result;
+ * other nb_inner values always use one launch per block column.  Applies to
+ * later calls on this handle. */
+rmh_status rmh_set_potrf_panel(rmh_handle h, int persistent);
 /* Look-ahead (mode 1, the default): rmh_?potrf factors panel k+1 on a
  * second, handle-owned stream while the caller's stream applies the rest of
  * panel k's trailing update; the call still completes in the caller's stream

@@ -58,6 +58,12 @@ SIGNATURES = {
     "rmh_strsm": (ST, [P, I, I, I, I, S, S, F, P, S, P, S, P]),
     "rmh_dtrsv": (ST, [P, I, I, I, S, P, S, P, S, P]),
     "rmh_strsv": (ST, [P, I, I, I, S, P, S, P, S, P]),
+    "rmh_dcopy_matrix": (ST, [P, I, S, S, P, S, P, S, P]),
+    "rmh_scopy_matrix": (ST, [P, I, S, S, P, S, P, S, P]),
+    "rmh_dtrmv": (ST, [P, I, I, I, S, P, S, P, S, P]),
+    "rmh_strmv": (ST, [P, I, I, I, S, P, S, P, S, P]),
+    "rmh_dtrmm": (ST, [P, I, I, I, I, S, S, D, P, S, P, S, P]),
+    "rmh_strmm": (ST, [P, I, I, I, I, S, S, F, P, S, P, S, P]),
     "rmh_dgemv": (ST, [P, I, S, S, D, P, S, P, S, D, P, S, P]),
     "rmh_sgemv": (ST, [P, I, S, S, F, P, S, P, S, F, P, S, P]),
     "rmh_dpotrf": (ST, [P, I, S, P, S, P, P]),
@@ -83,6 +89,7 @@ SIGNATURES = {
     "rmh_batched_autotune": (ST, [P, I, S, P, S, P, C.POINTER(D)]),
     "rmh_set_potrf_blocking": (ST, [P, S, S]),
     "rmh_set_potrf_lookahead": (ST, [P, I]),
+    "rmh_set_potrf_panel": (ST, [P, I]),
     "rmh_set_spotrf_precision": (ST, [P, I]),
     "rmh_profile_enable": (ST, [P, I]),
     "rmh_profile_reset": (ST, [P]),

@@ -326,13 +326,22 @@ rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
     const int64_t nb = (int64_t)(h->nb != 0 ? h->nb : (sizeof(T) == 8 ? Context::kDefaultNbF64 : Context::kDefaultNbF32));
     const int64_t nbi = (int64_t)h->nbi;
     // workspace: [0, 256) info when the caller passes none, then the side
-    // buffer of the diagonal-block factors (n x nbi, ld n)
-    RMH_TRY(h->reserve(256 + (size_t)std::max<int64_t>(n, 1) * nbi * sizeof(T)));
+    // buffer of the diagonal-block factors (n x nbi, ld n), then for the
+    // persistent panel kernel one counter block per panel (a ticket and a
+    // progress word per workgroup), all zeroed here
+    const bool persistent = h->persistent_panel && nbi == 32;
+    const int64_t npan = (n + nb - 1) / std::max<int64_t>(nb, 1);
+    const int64_t cstride = (1 + (n + 63) / 64 + 1) / 2 * 2;  // unsigned words per panel, even
+    const size_t dfac_bytes = ((size_t)std::max<int64_t>(n, 1) * nbi * sizeof(T) + 255) / 256 * 256;
+    const size_t ctr_bytes = persistent ? (size_t)npan * cstride * sizeof(unsigned) : 0;
+    RMH_TRY(h->reserve(256 + dfac_bytes + ctr_bytes));
     int64_t* scratch = static_cast<int64_t*>(h->ws);
     int64_t* info = d_info != nullptr ? d_info : scratch;
     T* dfac = reinterpret_cast<T*>(static_cast<char*>(h->ws) + 256);
+    unsigned* ctr = reinterpret_cast<unsigned*>(static_cast<char*>(h->ws) + 256 + dfac_bytes);
     RMH_TRY(launch_fill_i64(info, 1, 0, s));
     if (n == 0) return RMH_STATUS_SUCCESS;
+    if (persistent) RMH_TRY(launch_fill_i64(reinterpret_cast<int64_t*>(ctr), (int64_t)ctr_bytes / 8, 0, s));
     const int64_t rs = upper ? lda : 1, cs = upper ? 1 : lda;
     auto V = [&](int64_t i, int64_t j) { return A + i * rs + j * cs; };
     const bool lookahead = h->lookahead && n > 2 * nb;
@@ -341,6 +350,20 @@ rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
     // block column [j0, j0+jb): it applies the updates of the panel's earlier
     // columns [k0, j0), factors the diagonal block and solves the rows below.
     auto factor_panel = [&](int64_t k0, int64_t kw, hipStream_t st) -> rmh_status {
+        if (persistent) {
+            const int64_t rows = n - k0;
+            double flops = 0.0;
+            for (int64_t j0 = k0; j0 < k0 + kw; j0 += nbi) {
+                const int64_t jb = std::min(nbi, k0 + kw - j0), r = n - j0, K = j0 - k0;
+                flops += 2.0 * K * ((double)jb * (jb + 1) / 2.0 + (double)(r - jb) * jb) + (double)jb * jb * jb / 3.0 +
+                         (double)(r - jb) * jb * jb;
+            }
+            int slot;
+            RMH_TRY(h->prof_begin(st, PROF_PANEL, flops, &slot));
+            RMH_TRY(launch_potrf_panel_persistent<T>(V(k0, k0), rs, cs, rows, kw, info, k0, dfac + k0, n,
+                                                     ctr + (k0 / nb) * cstride, st));
+            return h->prof_end(st, slot);
+        }
         for (int64_t j0 = k0; j0 < k0 + kw; j0 += nbi) {
             const int64_t jb = std::min(nbi, k0 + kw - j0);
             const int64_t rows = n - j0, K = j0 - k0;
@@ -533,6 +556,42 @@ rmh_status trsv_entry(rmh_handle h, int upper, int trans, int unit, size_t n, co
 }
 
 template <typename T>
+rmh_status copy_matrix_entry(rmh_handle h, int trans, size_t m, size_t n, const T* A, size_t lda, T* B, size_t ldb,
+                             rmh_stream_t stream) {
+    RMH_CHECK_HANDLE(h);
+    RMH_REQUIRE(lda >= mx1(trans ? n : m), "copy_matrix: lda too small");
+    RMH_REQUIRE(ldb >= mx1(m), "copy_matrix: ldb too small");
+    if (m == 0 || n == 0) return RMH_STATUS_SUCCESS;
+    RMH_REQUIRE(A != nullptr && B != nullptr, "copy_matrix: NULL pointer");
+    return copy_matrix_impl<T>(trans != 0, (int64_t)m, (int64_t)n, A, (int64_t)lda, B, (int64_t)ldb, hs(stream));
+}
+
+template <typename T>
+rmh_status trmv_entry(rmh_handle h, int upper, int trans, int unit, size_t n, const T* A, size_t lda, T* x,
+                      size_t incx, rmh_stream_t stream) {
+    RMH_CHECK_HANDLE(h);
+    RMH_REQUIRE(lda >= mx1(n), "trmv: lda too small");
+    RMH_REQUIRE(incx >= 1, "trmv: incx must be positive");
+    if (n == 0) return RMH_STATUS_SUCCESS;
+    RMH_REQUIRE(A != nullptr && x != nullptr, "trmv: NULL pointer");
+    return trmv_impl<T>(ctx(h), upper != 0, trans != 0, unit != 0, (int64_t)n, A, (int64_t)lda, x, (int64_t)incx,
+                        hs(stream));
+}
+
+template <typename T>
+rmh_status trmm_entry(rmh_handle h, int left, int upper, int trans, int unit, size_t m, size_t n, T alpha,
+                      const T* A, size_t lda, T* B, size_t ldb, rmh_stream_t stream) {
+    RMH_CHECK_HANDLE(h);
+    const size_t na = left ? m : n;
+    RMH_REQUIRE(lda >= mx1(na), "trmm: lda too small");
+    RMH_REQUIRE(ldb >= mx1(m), "trmm: ldb too small");
+    if (m == 0 || n == 0) return RMH_STATUS_SUCCESS;
+    RMH_REQUIRE(A != nullptr && B != nullptr, "trmm: NULL pointer");
+    return trmm_impl<T>(ctx(h), left != 0, upper != 0, trans != 0, unit != 0, (int64_t)m, (int64_t)n, alpha, A,
+                        (int64_t)lda, B, (int64_t)ldb, hs(stream));
+}
+
+template <typename T>
 rmh_status gemv_entry(rmh_handle h, int trans, size_t m, size_t n, T alpha, const T* A, size_t lda, const T* x,
                       size_t incx, T beta, T* y, size_t incy, rmh_stream_t stream) {
     RMH_CHECK_HANDLE(h);
@@ -677,6 +736,30 @@ rmh_status rmh_dtrsv(rmh_handle h, int upper, int trans, int unit, size_t n, con
 rmh_status rmh_strsv(rmh_handle h, int upper, int trans, int unit, size_t n, const float* A, size_t lda, float* x,
                      size_t incx, rmh_stream_t stream) {
     return trsv_entry<float>(h, upper, trans, unit, n, A, lda, x, incx, stream);
+}
+rmh_status rmh_dcopy_matrix(rmh_handle h, int trans, size_t m, size_t n, const double* A, size_t lda, double* B,
+                            size_t ldb, rmh_stream_t stream) {
+    return copy_matrix_entry<double>(h, trans, m, n, A, lda, B, ldb, stream);
+}
+rmh_status rmh_scopy_matrix(rmh_handle h, int trans, size_t m, size_t n, const float* A, size_t lda, float* B,
+                            size_t ldb, rmh_stream_t stream) {
+    return copy_matrix_entry<float>(h, trans, m, n, A, lda, B, ldb, stream);
+}
+rmh_status rmh_dtrmv(rmh_handle h, int upper, int trans, int unit, size_t n, const double* A, size_t lda, double* x,
+                     size_t incx, rmh_stream_t stream) {
+    return trmv_entry<double>(h, upper, trans, unit, n, A, lda, x, incx, stream);
+}
+rmh_status rmh_strmv(rmh_handle h, int upper, int trans, int unit, size_t n, const float* A, size_t lda, float* x,
+                     size_t incx, rmh_stream_t stream) {
+    return trmv_entry<float>(h, upper, trans, unit, n, A, lda, x, incx, stream);
+}
+rmh_status rmh_dtrmm(rmh_handle h, int left, int upper, int trans, int unit, size_t m, size_t n, double alpha,
+                     const double* A, size_t lda, double* B, size_t ldb, rmh_stream_t stream) {
+    return trmm_entry<double>(h, left, upper, trans, unit, m, n, alpha, A, lda, B, ldb, stream);
+}
+rmh_status rmh_strmm(rmh_handle h, int left, int upper, int trans, int unit, size_t m, size_t n, float alpha,
+                     const float* A, size_t lda, float* B, size_t ldb, rmh_stream_t stream) {
+    return trmm_entry<float>(h, left, upper, trans, unit, m, n, alpha, A, lda, B, ldb, stream);
 }
 rmh_status rmh_dgemv(rmh_handle h, int trans, size_t m, size_t n, double alpha, const double* A, size_t lda,
                      const double* x, size_t incx, double beta, double* y, size_t incy, rmh_stream_t stream) {
@@ -831,6 +914,13 @@ rmh_status rmh_set_spotrf_precision(rmh_handle h, rmh_factor_precision prec) {
     if (h == nullptr) return fail(RMH_STATUS_INVALID_HANDLE, "NULL handle");
     RMH_REQUIRE(prec == RMH_FACTOR_FP32 || prec == RMH_FACTOR_BF16X3, "set_spotrf_precision: unknown precision");
     ctx(h)->spotrf_bf16x3 = prec == RMH_FACTOR_BF16X3;
+    return RMH_STATUS_SUCCESS;
+}
+
+rmh_status rmh_set_potrf_panel(rmh_handle h, int persistent) {
+    if (h == nullptr) return fail(RMH_STATUS_INVALID_HANDLE, "NULL handle");
+    RMH_REQUIRE(persistent == 0 || persistent == 1, "set_potrf_panel: 0 or 1");
+    ctx(h)->persistent_panel = persistent != 0;
     return RMH_STATUS_SUCCESS;
 }
 

@@ -42,6 +42,9 @@ struct Context {
     // split-BF16x3 kernel (set by rmh_dsposv for RMH_FACTOR_BF16X3)
     bool bf16x3_trailing = false;
     bool spotrf_bf16x3 = false;  // rmh_set_spotrf_precision
+    // 32-wide block columns: one persistent panel launch per panel instead of
+    // one launch per block column (rmh_set_potrf_panel)
+    bool persistent_panel = false;
     hipStream_t side = nullptr;  // look-ahead stream of potrf (default priority)
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     std::vector<ProfRec> recs;
@@ -95,6 +98,15 @@ rmh_status trsm_apply(const TrsmInverses<T>& inv, bool left, bool trans, int64_t
 template <typename T>
 rmh_status trsm_blocked(Context* h, bool left, bool upper, bool trans, bool unit, int64_t m, int64_t n, T alpha,
                         const T* A, int64_t lda, T* B, int64_t ldb, hipStream_t s);
+template <typename T>
+rmh_status copy_matrix_impl(bool trans, int64_t m, int64_t n, const T* A, int64_t lda, T* B, int64_t ldb,
+                            hipStream_t s);
+template <typename T>
+rmh_status trmv_impl(Context* h, bool upper, bool trans, bool unit, int64_t n, const T* A, int64_t lda, T* x,
+                     int64_t incx, hipStream_t s);
+template <typename T>
+rmh_status trmm_impl(Context* h, bool left, bool upper, bool trans, bool unit, int64_t m, int64_t n, T alpha,
+                     const T* A, int64_t lda, T* B, int64_t ldb, hipStream_t s);
 template <typename T>
 rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int64_t* d_info, hipStream_t s);
 template <typename T>

@@ -92,6 +92,31 @@ def trsv(h, A, x, upper=False, trans=False, unit=False, incx=1, stream=None):
     return x
 
 
+def copy_matrix(h, A, B, trans=False, stream=None):
+    """B = op(A) on the device (B column-major, m x n)."""
+    m, n = B.shape
+    f = _fn("rmh_", B.dtype, "copy_matrix")
+    capi.check(f(h.value, int(trans), m, n, _p(A), _ld(A), _p(B), _ld(B), _stream(stream)))
+    return B
+
+
+def trmv(h, A, x, upper=False, trans=False, unit=False, incx=1, stream=None):
+    """x = op(A) x in place, A triangular; x is a 1-D tensor read with stride incx."""
+    n = A.shape[0]
+    f = _fn("rmh_", A.dtype, "trmv")
+    capi.check(f(h.value, int(upper), int(trans), int(unit), n, _p(A), _ld(A), _p(x), incx, _stream(stream)))
+    return x
+
+
+def trmm(h, A, B, left=True, upper=False, trans=False, unit=False, alpha=1.0, stream=None):
+    """B = alpha op(A) B (left) or alpha B op(A) (right) in place, A triangular."""
+    m, n = B.shape
+    f = _fn("rmh_", B.dtype, "trmm")
+    capi.check(f(h.value, int(left), int(upper), int(trans), int(unit), m, n, alpha, _p(A), _ld(A), _p(B), _ld(B),
+                 _stream(stream)))
+    return B
+
+
 def gemv(h, A, x, y, alpha=1.0, beta=0.0, trans=False, incx=1, incy=1, stream=None):
     m, n = A.shape
     f = _fn("rmh_", A.dtype, "gemv")
@@ -241,6 +266,12 @@ def set_potrf_blocking(h, nb=0, nb_inner=0):
 def set_spotrf_precision(h, prec="fp32"):
     p = {"fp32": capi.FACTOR_FP32, "bf16x3": capi.FACTOR_BF16X3}[prec]
     capi.check(capi.load().rmh_set_spotrf_precision(h.value, p))
+
+
+def set_potrf_panel(h, persistent=True):
+    """32-wide diagonal blocks: one persistent panel launch per panel (True,
+    the default) or one launch per block column (False)."""
+    capi.check(capi.load().rmh_set_potrf_panel(h.value, int(persistent)))
 
 
 def set_potrf_lookahead(h, enable=True):

@@ -13,7 +13,7 @@ OUT = os.path.join(HERE, "test_frontend")
 
 
 def build(verbose: bool = False) -> str:
-    cmd = [os.environ.get("HIPCC", "/opt/rocm/bin/hipcc"), "-std=c++17", "-O2", "-Wall",
+    cmd = [os.environ.get("HIPCC", "/opt/rocm/bin/hipcc"), "--offload-arch=gfx950", "-std=c++17", "-O2", "-Wall",
            os.path.join(HERE, "test_frontend.cpp"), "-I", os.path.join(ROOT, "include"),
            "-L", os.path.join(ROOT, "remora_amd", "lib"), "-lremora_hip",
            "-L", os.path.join(ROOT, "oracle"), "-loracle",

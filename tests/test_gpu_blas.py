@@ -288,3 +288,61 @@ def test_dtrsm_many_rhs_wide_leaves(handle, left, upper, trans, unit):
     resid = (opT @ X - 0.5 * B) if left else (X @ opT - 0.5 * B)
     scale = (np.abs(opT) @ np.abs(X)) if left else (np.abs(X) @ np.abs(opT))
     assert np.max(np.abs(resid) / scale) < 1e-12
+
+
+TRMM = [(left, upper, trans, unit) for left in (True, False) for upper in (False, True) for trans in (False, True)
+        for unit in (False, True)]
+
+
+@pytest.mark.parametrize("left,upper,trans,unit", TRMM)
+@pytest.mark.parametrize("m,n", [(45, 70), (700, 300)])
+def test_dtrmm_exact_on_integers(handle, left, upper, trans, unit, m, n):
+    """B = alpha op(T) B or alpha B op(T) (blas_device::trmm) with alpha = -2:
+    small integers make every partial sum exact, so the device result equals
+    the closed form exactly; the other stored triangle (NaN) and a unit
+    diagonal (stored as 7) are never read."""
+    from remora_amd import linalg as la
+    na = m if left else n
+    rng = np.random.default_rng(m + 3 * n + 16 * left + 8 * upper + 4 * trans + 2 * unit)
+    A = rng.integers(-3, 4, (na, na)).astype(float)
+    T = np.triu(A) if upper else np.tril(A)
+    S = T.copy()
+    S[np.tril_indices(na, -1) if upper else np.triu_indices(na, 1)] = np.nan
+    if unit:
+        T[np.diag_indices(na)] = 1.0
+        S[np.diag_indices(na)] = 7.0
+    B = rng.integers(-3, 4, (m, n)).astype(float)
+    opT = T.T if trans else T
+    ref = -2.0 * (opT @ B if left else B @ opT)
+    dB = dev(B, ld=m + 5)
+    la.trmm(handle, dev(S), dB, left=left, upper=upper, trans=trans, unit=unit, alpha=-2.0)
+    assert np.array_equal(host(dB), ref)
+
+
+def test_strmm_rounding(handle):
+    import torch
+    from remora_amd import linalg as la
+    m, n = 500, 260
+    A = synth.uniform(12, 0, m * m, -1, 1).reshape(m, m)
+    B = synth.uniform(12, 1, m * n, -1, 1).reshape(m, n)
+    dB = dev(B, torch.float32)
+    la.trmm(handle, dev(A, torch.float32), dB, left=True, upper=True, trans=True, alpha=0.5)
+    ref = 0.5 * np.triu(A).T @ B
+    scale = 0.5 * np.abs(np.triu(A).T) @ np.abs(B)
+    assert np.all(np.abs(host(dB) - ref) <= 2e-5 * scale + 1e-30)
+
+
+@pytest.mark.parametrize("trans", [False, True])
+@pytest.mark.parametrize("m,n", [(1, 1), (63, 65), (130, 1000), (4097, 129)])
+def test_copy_matrix_exact(handle, trans, m, n):
+    """B = op(A) bit for bit; B's padding rows (ld > m) are never written."""
+    import torch
+    from remora_amd import linalg as la
+    A = synth.normal(m + n, 0, m * n).reshape((n, m) if trans else (m, n))
+    dB = dev(np.zeros((m, n)), ld=m + 7)
+    buf = dB.t()  # (n, ld) view of the whole buffer, to look at the padding
+    la.copy_matrix(handle, dev(A, ld=A.shape[0] + 3), dB, trans=trans)
+    torch.cuda.synchronize()
+    assert np.array_equal(host(dB), A.T if trans else A)
+    pad = buf.as_strided((n, 7), (m + 7, 1), storage_offset=buf.storage_offset() + m)
+    assert bool(torch.isnan(pad).all())
