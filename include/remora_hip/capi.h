@@ -293,12 +293,6 @@ rmh_status rmh_set_potrf_blocking(rmh_handle h, size_t nb, size_t nb_inner);
  * RMH_FACTOR_FP32 (default, FP32 MFMA) or RMH_FACTOR_BF16X3 (split-BF16x3,
  * lower storage).  rmh_dsposv sets its own choice for its factorisation. */
 rmh_status rmh_set_spotrf_precision(rmh_handle h, rmh_factor_precision prec);
-/* Panel kernel of rmh_?potrf for 32-wide diagonal blocks (nb_inner 32, the
- * default): 1 (the default) factors each panel in one persistent launch,
- * 0 in one launch per 32-column block column.  Both give the same result;
- * other nb_inner values always use one launch per block column.  Applies to
- * later calls on this handle. */
-rmh_status rmh_set_potrf_panel(rmh_handle h, int persistent);
 /* Look-ahead (mode 1, the default): rmh_?potrf factors panel k+1 on a
  * second, handle-owned stream while the caller's stream applies the rest of
  * panel k's trailing update; the call still completes in the caller's stream

@@ -89,7 +89,6 @@ SIGNATURES = {
     "rmh_batched_autotune": (ST, [P, I, S, P, S, P, C.POINTER(D)]),
     "rmh_set_potrf_blocking": (ST, [P, S, S]),
     "rmh_set_potrf_lookahead": (ST, [P, I]),
-    "rmh_set_potrf_panel": (ST, [P, I]),
     "rmh_set_spotrf_precision": (ST, [P, I]),
     "rmh_profile_enable": (ST, [P, I]),
     "rmh_profile_reset": (ST, [P]),

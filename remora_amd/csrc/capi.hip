@@ -326,22 +326,13 @@ rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
     const int64_t nb = (int64_t)(h->nb != 0 ? h->nb : (sizeof(T) == 8 ? Context::kDefaultNbF64 : Context::kDefaultNbF32));
     const int64_t nbi = (int64_t)h->nbi;
     // workspace: [0, 256) info when the caller passes none, then the side
-    // buffer of the diagonal-block factors (n x nbi, ld n), then for the
-    // persistent panel kernel one counter block per panel (a ticket and a
-    // progress word per workgroup), all zeroed here
-    const bool persistent = h->persistent_panel && nbi == 32;
-    const int64_t npan = (n + nb - 1) / std::max<int64_t>(nb, 1);
-    const int64_t cstride = (1 + (n + 63) / 64 + 1) / 2 * 2;  // unsigned words per panel, even
-    const size_t dfac_bytes = ((size_t)std::max<int64_t>(n, 1) * nbi * sizeof(T) + 255) / 256 * 256;
-    const size_t ctr_bytes = persistent ? (size_t)npan * cstride * sizeof(unsigned) : 0;
-    RMH_TRY(h->reserve(256 + dfac_bytes + ctr_bytes));
+    // buffer of the diagonal-block factors (n x nbi, ld n)
+    RMH_TRY(h->reserve(256 + (size_t)std::max<int64_t>(n, 1) * nbi * sizeof(T)));
     int64_t* scratch = static_cast<int64_t*>(h->ws);
     int64_t* info = d_info != nullptr ? d_info : scratch;
     T* dfac = reinterpret_cast<T*>(static_cast<char*>(h->ws) + 256);
-    unsigned* ctr = reinterpret_cast<unsigned*>(static_cast<char*>(h->ws) + 256 + dfac_bytes);
     RMH_TRY(launch_fill_i64(info, 1, 0, s));
     if (n == 0) return RMH_STATUS_SUCCESS;
-    if (persistent) RMH_TRY(launch_fill_i64(reinterpret_cast<int64_t*>(ctr), (int64_t)ctr_bytes / 8, 0, s));
     const int64_t rs = upper ? lda : 1, cs = upper ? 1 : lda;
     auto V = [&](int64_t i, int64_t j) { return A + i * rs + j * cs; };
     const bool lookahead = h->lookahead && n > 2 * nb;
@@ -350,20 +341,6 @@ rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
     // block column [j0, j0+jb): it applies the updates of the panel's earlier
     // columns [k0, j0), factors the diagonal block and solves the rows below.
     auto factor_panel = [&](int64_t k0, int64_t kw, hipStream_t st) -> rmh_status {
-        if (persistent) {
-            const int64_t rows = n - k0;
-            double flops = 0.0;
-            for (int64_t j0 = k0; j0 < k0 + kw; j0 += nbi) {
-                const int64_t jb = std::min(nbi, k0 + kw - j0), r = n - j0, K = j0 - k0;
-                flops += 2.0 * K * ((double)jb * (jb + 1) / 2.0 + (double)(r - jb) * jb) + (double)jb * jb * jb / 3.0 +
-                         (double)(r - jb) * jb * jb;
-            }
-            int slot;
-            RMH_TRY(h->prof_begin(st, PROF_PANEL, flops, &slot));
-            RMH_TRY(launch_potrf_panel_persistent<T>(V(k0, k0), rs, cs, rows, kw, info, k0, dfac + k0, n,
-                                                     ctr + (k0 / nb) * cstride, st));
-            return h->prof_end(st, slot);
-        }
         for (int64_t j0 = k0; j0 < k0 + kw; j0 += nbi) {
             const int64_t jb = std::min(nbi, k0 + kw - j0);
             const int64_t rows = n - j0, K = j0 - k0;
@@ -914,13 +891,6 @@ rmh_status rmh_set_spotrf_precision(rmh_handle h, rmh_factor_precision prec) {
     if (h == nullptr) return fail(RMH_STATUS_INVALID_HANDLE, "NULL handle");
     RMH_REQUIRE(prec == RMH_FACTOR_FP32 || prec == RMH_FACTOR_BF16X3, "set_spotrf_precision: unknown precision");
     ctx(h)->spotrf_bf16x3 = prec == RMH_FACTOR_BF16X3;
-    return RMH_STATUS_SUCCESS;
-}
-
-rmh_status rmh_set_potrf_panel(rmh_handle h, int persistent) {
-    if (h == nullptr) return fail(RMH_STATUS_INVALID_HANDLE, "NULL handle");
-    RMH_REQUIRE(persistent == 0 || persistent == 1, "set_potrf_panel: 0 or 1");
-    ctx(h)->persistent_panel = persistent != 0;
     return RMH_STATUS_SUCCESS;
 }
 

@@ -380,250 +380,6 @@ __global__ __launch_bounds__(panel_threads(NB)) void potrf_panel_kernel(T* P, in
     }
 }
 
-// Persistent form of the 32-wide panel step: one launch factors a whole
-// panel of width kw (all its 32-column block columns) instead of one launch
-// per block column.  The arithmetic of every element is that of
-// potrf_panel_kernel<T, 32> (same MFMA chunks and K-group order, same
-// elimination), so the two forms give identical results.
-//
-// Workgroups take a ticket from ctr[0] when they start, and ticket w owns the
-// panel rows [64w, 64w+64) for the whole launch; it takes part in the block
-// columns j whose diagonal row 32j lies above its last row.  Block column j
-// needs the rows of its diagonal block, D_j = V(32j : 32j+32, 0 : 32j), which
-// belong to ticket o = j/2 (lower than or equal to w): o publishes in
-// ctr[1 + o] how many block columns it has completed.  A workgroup starts
-// the update of its rows with the columns of D_j that o completed up to
-// block column j-2 (progress >= j-1), then waits for the last 32 (progress
-// >= j).  Only lower tickets are waited on, and they were dispatched earlier,
-// so the waits end; each is bounded all the same (kSpinLimit polls, about a
-// second), after which *info is set to -1 and the workgroup stops.
-// A failed pivot is seen by every workgroup of that block column (each forms
-// the same factor), so all stop at the same step; the owner records it.
-constexpr unsigned kSpinLimit = 1u << 24;
-
-template <typename T>
-__global__ __launch_bounds__(512) void potrf_panel_persistent_kernel(T* P0, int64_t rs, int64_t cs, int64_t m, int kw,
-                                                                     int64_t* info, int64_t info_base, T* dfac0,
-                                                                     int64_t ldf, unsigned* ctr) {
-    constexpr int NB = 32;
-    constexpr int kThreads = 512;
-    constexpr int KG = 2;                     // wavefronts sharing a row block (K groups)
-    constexpr int NCB = NB / 16;              // 16-column blocks
-    constexpr int NDB = NCB * (NCB + 1) / 2;  // lower blocks of the diagonal block
-    constexpr int LD = NB + 1;
-    constexpr int CD = (NB * NB + kThreads - 1) / kThreads, CR = kRW * NB / kThreads;
-    constexpr int DEP = 2;
-    constexpr int OPW = 64 - NB;  // own rows per factoring wavefront
-    constexpr int NFW = (kRW + OPW - 1) / OPW;
-
-    __shared__ double L[NB * LD];
-    __shared__ double X[kRW * LD];
-    __shared__ __align__(16) double col[4 * NB];
-    __shared__ int s_w, s_bad, s_abort;
-
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int wr = wave & 3, kg = wave >> 2;
-    const int li = lane & 15, lk = lane >> 4;
-    if (tid == 0) {
-        s_w = (int)atomicAdd(&ctr[0], 1u);
-        s_abort = 0;
-    }
-    __syncthreads();
-    const int w = s_w;
-    unsigned* prog = ctr + 1;
-    const int nsteps = (kw + NB - 1) / NB;
-    const int last = min(nsteps - 1, 2 * w + 1);  // block columns with a diagonal row above this workgroup's last row
-
-    // this wavefront's diagonal block of D D^T and the order it holds D's
-    // column blocks in (as in potrf_panel_kernel<T, 32>)
-    int pblk[NCB];
-#pragma unroll
-    for (int b = 0; b < NCB; ++b) pblk[b] = b;
-    if (wr == 1 || wr == 2) { pblk[0] = 1; pblk[NCB - 1] = 0; }
-    const int dm = wr >= NDB ? 0 : (wr == 1 ? 2 : 1);
-    int dbi = 0, dbj = 0;
-    if (wr < NDB) lower_block(wr, dbi, dbj);
-
-    // poll ctr[1 + o] until it reaches target; workgroup-uniform result
-    auto wait_for = [&](int o, unsigned target) -> bool {
-        if (tid == 0) {
-            unsigned spins = 0;
-            while (__hip_atomic_load(&prog[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-                __builtin_amdgcn_s_sleep(2);
-                if (++spins == kSpinLimit) {
-                    s_abort = 1;
-                    break;
-                }
-            }
-        }
-        __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        return s_abort == 0;
-    };
-    auto abort_all = [&]() {
-        if (tid == 0) atomicCAS(reinterpret_cast<unsigned long long*>(info), 0ull, ~0ull);  // info = -1
-    };
-
-    for (int j = 0; j <= last; ++j) {
-        const int64_t d0 = (int64_t)NB * j;
-        const int kb = (int)min((int64_t)NB, (int64_t)kw - d0);
-        const int K = (int)d0;
-        T* P = P0 + d0 * rs;  // V(i, c): row d0 + i of the panel, panel column c
-        const int64_t rows = m - d0;
-        const int o = j / 2;  // owner of the diagonal rows
-        const int64_t r0 = max((int64_t)kRW * w, d0 + kb) - d0;  // first own row, relative to the diagonal
-        const int nown = (int)max((int64_t)0, min((int64_t)kRW * w + kRW, m) - d0 - r0);
-        const int nch = K / kKc;                 // K is a multiple of 32: no partial chunk
-        const int ncha = max(K - NB, 0) / kKc;   // chunks before the last 32 columns
-
-        // block column: original values of the own rows and the diagonal block
-        T cur_r[CR], cur_d[CD];
-#pragma unroll
-        for (int q = 0; q < CR; ++q) {
-            const int e = tid + q * kThreads, i = e % kRW, c = e / kRW;
-            cur_r[q] = P[(i < nown ? r0 + i : 0) * rs + (int64_t)(K + (c < kb ? c : 0)) * cs];
-        }
-#pragma unroll
-        for (int q = 0; q < CD; ++q) {
-            const int e = min(tid + q * kThreads, NB * NB - 1), i = e % NB, c = e / NB;
-            const bool ok = i < kb && c <= i;
-            cur_d[q] = P[(ok ? i : 0) * rs + (int64_t)(K + (ok ? c : 0)) * cs];
-        }
-        if (tid == 0) s_bad = -1;
-
-        const bool a_ok = 16 * wr + li < nown;
-        const T* pa = P + (a_ok ? r0 + 16 * wr + li : 0) * rs;
-        const T* pb[NCB];
-#pragma unroll
-        for (int b = 0; b < NCB; ++b) pb[b] = P + (int64_t)(16 * pblk[b] + li < kb ? 16 * pblk[b] + li : 0) * rs;
-
-        dx4 acc[NCB], dacc = dx4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int b = 0; b < NCB; ++b) acc[b] = dx4{0.0, 0.0, 0.0, 0.0};
-        T av[DEP][4], bv[DEP][4][NCB];
-        auto load_chunk = [&](int ch, int kmax, T(&a)[4], T(&bm)[4][NCB]) {
-#pragma unroll
-            for (int s4 = 0; s4 < 4; ++s4) {
-                const int64_t off = (int64_t)min(ch * kKc + 4 * s4 + lk, kmax) * cs;
-                a[s4] = pa[off];
-#pragma unroll
-                for (int b = 0; b < NCB; ++b) bm[s4][b] = pb[b][off];
-            }
-        };
-        auto compute = [&](auto dm_c, const T(&a)[4], const T(&bm)[4][NCB]) {
-            constexpr int DM = decltype(dm_c)::value;
-#pragma unroll
-            for (int s4 = 0; s4 < 4; ++s4) {
-                const double av4 = (double)a[s4];
-                double bv4[NCB];
-#pragma unroll
-                for (int b = 0; b < NCB; ++b) bv4[b] = (double)bm[s4][b];
-#pragma unroll
-                for (int b = 0; b < NCB; ++b)
-                    acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av4, bv4[b], acc[b], 0, 0, 0);
-                if constexpr (DM == 1 || DM == 2)
-                    dacc = __builtin_amdgcn_mfma_f64_16x16x4f64(bv4[0], bv4[DM == 2 ? NCB - 1 : 0], dacc, 0, 0, 0);
-            }
-        };
-        // chunks [lo, hi) of this K group (lo even), loads clamped to depth kmax
-        auto kloop = [&](auto dm_c, int lo, int hi, int kmax) {
-            if (lo + kg >= hi) return;
-            load_chunk(lo + kg, kmax, av[0], bv[0]);
-            for (int t0 = 0; lo + kg + KG * t0 < hi; t0 += DEP) {
-#pragma unroll
-                for (int s = 0; s < DEP; ++s) {
-                    const int ch = lo + kg + KG * (t0 + s);
-                    if (ch >= hi) break;
-                    load_chunk(ch + KG * (DEP - 1), kmax, av[(s + DEP - 1) % DEP], bv[(s + DEP - 1) % DEP]);
-                    compute(dm_c, av[s], bv[s]);
-                }
-            }
-        };
-        auto kphase = [&](int lo, int hi, int kmax) {
-            if (dm == 1) kloop(std::integral_constant<int, 1>{}, lo, hi, kmax);
-            else if (dm == 2) kloop(std::integral_constant<int, 2>{}, lo, hi, kmax);
-            else kloop(std::integral_constant<int, 0>{}, lo, hi, kmax);
-        };
-
-        // phase A: the columns of D_j complete up to block column j-2
-        if (o != w && j >= 2 && !wait_for(o, (unsigned)(j - 1))) { abort_all(); return; }
-        kphase(0, ncha, max(NB * (j - 1), 1) - 1);
-        // phase B: the last 32 columns
-        if (o != w && j >= 1 && !wait_for(o, (unsigned)j)) { abort_all(); return; }
-        kphase(ncha, nch, max(K, 1) - 1);
-
-#pragma unroll
-        for (int q = 0; q < CR; ++q) {
-            const int e = tid + q * kThreads, i = e % kRW, c = e / kRW;
-            X[i * LD + c] = (i < nown && c < kb) ? (double)cur_r[q] : 0.0;
-        }
-#pragma unroll
-        for (int q = 0; q < CD; ++q) {
-            const int e = tid + q * kThreads, i = e % NB, c = e / NB;
-            if (e < NB * NB) L[i * LD + c] = (i < kb && c <= i) ? (double)cur_d[q] : (i == c ? 1.0 : 0.0);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int g = 0; g < KG; ++g) {
-            if (K > 0 && kg == g) {
-#pragma unroll
-                for (int b = 0; b < NCB; ++b)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) X[(16 * wr + 4 * r + lk) * LD + 16 * pblk[b] + li] -= acc[b][r];
-                if (wr < NDB) {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int i = 16 * dbi + 4 * r + lk, c = 16 * dbj + li;
-                        if (c <= i && i < kb) L[i * LD + c] -= dacc[r];
-                    }
-                }
-            }
-            __syncthreads();
-        }
-
-        // factor the diagonal block and solve the own rows along with it
-        if (wave < NFW) {
-            const int orow = wave * OPW + lane - NB;
-            const bool is_diag = lane < NB, is_row = !is_diag && orow < kRW;
-            const double* src = is_diag ? &L[lane * LD] : &X[(is_row ? orow : 0) * LD];
-            double r[NB];
-#pragma unroll
-            for (int c = 0; c < NB; ++c) r[c] = src[c];
-            const int bad = wave_chol_rows<NB>(r, &col[wave * 2 * NB], lane);
-            if (wave == 0 && lane == 0) s_bad = bad;
-            if (bad < 0) {
-                if (is_row && orow < nown) {
-                    const int64_t i = r0 + orow;
-#pragma unroll
-                    for (int c = 0; c < NB; ++c)
-                        if (c < kb) P[i * rs + (int64_t)(K + c) * cs] = (T)r[c];
-                }
-                if (w == o && wave == 0 && lane < kb) {
-                    T* df = dfac0 + d0;
-#pragma unroll
-                    for (int c = 0; c < NB; ++c)
-                        if (c <= lane) df[lane + c * ldf] = (T)r[c];
-                }
-            }
-        }
-        __syncthreads();
-        if (s_bad >= 0) {
-            if (w == o && tid == 0)
-                atomicCAS(reinterpret_cast<unsigned long long*>(info), 0ull,
-                          (unsigned long long)(info_base + d0 + s_bad + 1));
-            return;
-        }
-        // publish the steps other workgroups wait for: the diagonal rows of
-        // block columns 2w and 2w+1 are this workgroup's
-        if (j >= 2 * w - 2 && j <= 2 * w) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            __syncthreads();
-            if (tid == 0) __hip_atomic_store(&prog[w], (unsigned)(j + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
-}
-
 }  // namespace
 
 // Copy the diagonal-block factors from the side buffer into the matrix:
@@ -667,19 +423,6 @@ rmh_status launch_potrf_panel(T* P, int64_t rs, int64_t cs, int64_t rows, int64_
 }
 
 template <typename T>
-rmh_status launch_potrf_panel_persistent(T* P, int64_t rs, int64_t cs, int64_t rows, int64_t kw, int64_t* info,
-                                         int64_t info_base, T* dfac, int64_t ldf, unsigned* ctr, hipStream_t stream) {
-    if (kw <= 0) return RMH_STATUS_SUCCESS;
-    if (rows < kw) return fail(RMH_STATUS_INVALID_VALUE, "potrf_panel: fewer rows than columns");
-    if (kw > 1 << 20) return fail(RMH_STATUS_INVALID_VALUE, "potrf_panel: panel too wide");
-    const int64_t grid = (rows + kRW - 1) / kRW;
-    if (grid > (int64_t)1 << 24) return fail(RMH_STATUS_INVALID_VALUE, "potrf_panel: too many rows");
-    hipLaunchKernelGGL((potrf_panel_persistent_kernel<T>), dim3((unsigned)grid), dim3(512), 0, stream, P, rs, cs,
-                       rows, (int)kw, info, info_base, dfac, ldf, ctr);
-    return check_launch("potrf_panel_persistent_kernel");
-}
-
-template <typename T>
 rmh_status launch_scatter_diag(T* A, int64_t rs, int64_t cs, int64_t n, int nbi, const T* dfac, int64_t ldf,
                                const int64_t* info, hipStream_t stream) {
     if (n <= 0) return RMH_STATUS_SUCCESS;
@@ -692,10 +435,6 @@ template rmh_status launch_potrf_panel<double>(double*, int64_t, int64_t, int64_
                                                double*, int64_t, hipStream_t);
 template rmh_status launch_potrf_panel<float>(float*, int64_t, int64_t, int64_t, int64_t, int, int64_t*, int64_t,
                                               float*, int64_t, hipStream_t);
-template rmh_status launch_potrf_panel_persistent<double>(double*, int64_t, int64_t, int64_t, int64_t, int64_t*,
-                                                          int64_t, double*, int64_t, unsigned*, hipStream_t);
-template rmh_status launch_potrf_panel_persistent<float>(float*, int64_t, int64_t, int64_t, int64_t, int64_t*, int64_t,
-                                                         float*, int64_t, unsigned*, hipStream_t);
 template rmh_status launch_scatter_diag<double>(double*, int64_t, int64_t, int64_t, int, const double*, int64_t,
                                                 const int64_t*, hipStream_t);
 template rmh_status launch_scatter_diag<float>(float*, int64_t, int64_t, int64_t, int, const float*, int64_t,

@@ -42,9 +42,6 @@ struct Context {
     // split-BF16x3 kernel (set by rmh_dsposv for RMH_FACTOR_BF16X3)
     bool bf16x3_trailing = false;
     bool spotrf_bf16x3 = false;  // rmh_set_spotrf_precision
-    // 32-wide block columns: one persistent panel launch per panel instead of
-    // one launch per block column (rmh_set_potrf_panel)
-    bool persistent_panel = false;
     hipStream_t side = nullptr;  // look-ahead stream of potrf (default priority)
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     std::vector<ProfRec> recs;

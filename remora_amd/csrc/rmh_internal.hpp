@@ -104,16 +104,6 @@ template <typename T>
 rmh_status launch_potrf_panel(T* P, int64_t rs, int64_t cs, int64_t rows, int64_t K, int kb, int64_t* info,
                               int64_t info_base, T* dfac, int64_t ldf, hipStream_t stream);
 
-// The whole panel V[0:rows, 0:kw] (V(0, 0) its diagonal corner) in one
-// launch of ceil(rows / 64) workgroups, 32-wide block columns, the same
-// arithmetic as kw/32 launch_potrf_panel calls; factors of the diagonal
-// blocks go to dfac as there (row r of the panel at dfac[r + c*ldf]).  ctr
-// holds 1 + ceil(rows / 64) zeroed words (a ticket counter and one progress
-// word per workgroup) and must be zeroed again before the next launch.
-template <typename T>
-rmh_status launch_potrf_panel_persistent(T* P, int64_t rs, int64_t cs, int64_t rows, int64_t kw, int64_t* info,
-                                         int64_t info_base, T* dfac, int64_t ldf, unsigned* ctr, hipStream_t stream);
-
 // Copy the diagonal-block factors written by launch_potrf_panel (blocks of
 // nbi rows; block j0's element (i, c) at dfac[j0 + i + c*ldf]) into V of
 // the n x n matrix; blocks at or after a failed pivot (*info != 0) are left.

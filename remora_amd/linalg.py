@@ -268,12 +268,6 @@ def set_spotrf_precision(h, prec="fp32"):
     capi.check(capi.load().rmh_set_spotrf_precision(h.value, p))
 
 
-def set_potrf_panel(h, persistent=True):
-    """32-wide diagonal blocks: one persistent panel launch per panel (True,
-    the default) or one launch per block column (False)."""
-    capi.check(capi.load().rmh_set_potrf_panel(h.value, int(persistent)))
-
-
 def set_potrf_lookahead(h, enable=True):
     """enable: False / True, or the mode 0 (off), 1 (on), 2 (on, trailing
     update one workgroup per compute unit)."""
