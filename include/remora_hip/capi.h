@@ -187,6 +187,20 @@ rmh_status rmh_dchol_update(rmh_handle h, int upper, size_t n, double alpha, dou
 rmh_status rmh_schol_update(rmh_handle h, int upper, size_t n, float alpha, float beta, float* L, size_t ldl,
                             const float* v, size_t incv, int64_t* d_info, rmh_stream_t stream);
 
+/* rmh_dpotrf without any allocation: the same factorisation, arguments and
+ * result (bit for bit), on a static per-device scratch array of the library
+ * instead of the handle's workspace, on the caller's stream alone (no
+ * look-ahead stream, no profile events).  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Limit:
+ * n <= 32768 with the default nb_inner 32 (n <= 16384 at nb_inner 64; the
+ * scratch is 8 MiB + 256 bytes per GPU, held until the process ends); larger
+ * orders return RMH_STATUS_INVALID_VALUE.  Every call on one GPU shares that
+ * GPU's scratch: calls that overlap on one GPU race and must be ordered by
+ * the caller.  Asynchronous, like rmh_dpotrf. */
+rmh_status rmh_dpotrf_noalloc(rmh_handle h, int upper, size_t n, double* A, size_t lda, int64_t* d_info,
+                              rmh_stream_t stream);
+
 /* ----------------------------------------------------- mixed precision -- */
 /* Precision of the factorisation inside rmh_dsposv. */
 typedef enum rmh_factor_precision {

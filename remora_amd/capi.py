@@ -68,6 +68,7 @@ SIGNATURES = {
     "rmh_sgemv": (ST, [P, I, S, S, F, P, S, P, S, F, P, S, P]),
     "rmh_dpotrf": (ST, [P, I, S, P, S, P, P]),
     "rmh_spotrf": (ST, [P, I, S, P, S, P, P]),
+    "rmh_dpotrf_noalloc": (ST, [P, I, S, P, S, P, P]),
     "rmh_dpotrs": (ST, [P, I, S, S, P, S, P, S, P]),
     "rmh_spotrs": (ST, [P, I, S, S, P, S, P, S, P]),
     "rmh_dsposv": (ST, [P, I, I, S, S, P, S, P, S, P, S, D, I, C.POINTER(I), C.POINTER(D), P]),

@@ -322,20 +322,32 @@ static rmh_status launch_trailing(Context* h, const GemmDesc<T>& g, Tile tile, h
 }
 
 template <typename T>
-rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int64_t* d_info, hipStream_t s) {
+size_t potrf_workspace(const Context* h, int64_t n) {
+    return 256 + (size_t)std::max<int64_t>(n, 1) * h->nbi * sizeof(T);
+}
+
+// The factorisation on a caller-provided workspace of potrf_workspace<T>(h, n)
+// bytes: [0, 256) info when the caller passes none, then the side buffer of
+// the diagonal-block factors (n x nbi, ld n).  With `plain` the call runs on
+// the caller's stream alone and records no profile events, so it creates and
+// allocates nothing.
+template <typename T>
+rmh_status potrf_core(Context* h, bool upper, int64_t n, T* A, int64_t lda, int64_t* d_info, void* ws, bool plain,
+                      hipStream_t s) {
     const int64_t nb = (int64_t)(h->nb != 0 ? h->nb : (sizeof(T) == 8 ? Context::kDefaultNbF64 : Context::kDefaultNbF32));
     const int64_t nbi = (int64_t)h->nbi;
-    // workspace: [0, 256) info when the caller passes none, then the side
-    // buffer of the diagonal-block factors (n x nbi, ld n)
-    RMH_TRY(h->reserve(256 + (size_t)std::max<int64_t>(n, 1) * nbi * sizeof(T)));
-    int64_t* scratch = static_cast<int64_t*>(h->ws);
+    int64_t* scratch = static_cast<int64_t*>(ws);
     int64_t* info = d_info != nullptr ? d_info : scratch;
-    T* dfac = reinterpret_cast<T*>(static_cast<char*>(h->ws) + 256);
+    T* dfac = reinterpret_cast<T*>(static_cast<char*>(ws) + 256);
     RMH_TRY(launch_fill_i64(info, 1, 0, s));
     if (n == 0) return RMH_STATUS_SUCCESS;
     const int64_t rs = upper ? lda : 1, cs = upper ? 1 : lda;
     auto V = [&](int64_t i, int64_t j) { return A + i * rs + j * cs; };
-    const bool lookahead = h->lookahead && n > 2 * nb;
+    const bool lookahead = !plain && h->lookahead && n > 2 * nb;
+    auto prof_begin = [&](hipStream_t st, int cls, double flops, int* slot) {
+        *slot = -1;
+        return plain ? RMH_STATUS_SUCCESS : h->prof_begin(st, cls, flops, slot);
+    };
 
     // Factor the panel V[k0:, k0:k0+kw] left-looking, one fused kernel per
     // block column [j0, j0+jb): it applies the updates of the panel's earlier
@@ -347,7 +359,7 @@ rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
             const double flops = 2.0 * K * ((double)jb * (jb + 1) / 2.0 + (double)(rows - jb) * jb) +
                                  (double)jb * jb * jb / 3.0 + (double)(rows - jb) * jb * jb;
             int slot;
-            RMH_TRY(h->prof_begin(st, PROF_PANEL, flops, &slot));
+            RMH_TRY(prof_begin(st, PROF_PANEL, flops, &slot));
             RMH_TRY(launch_potrf_panel<T>(V(j0, k0), rs, cs, rows, K, (int)jb, info, j0, dfac + j0, n, st));
             RMH_TRY(h->prof_end(st, slot));
         }
@@ -373,7 +385,7 @@ rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
         const Tile tile = pick_tile(g.M, g.N, g.part);
         g.one_per_cu = lookahead && h->share_cus && tile == Tile::T128x128;
         int slot;
-        RMH_TRY(h->prof_begin(st, tile == Tile::T128x128 ? PROF_TRAILING : PROF_TRAILING_SMALL, flops, &slot));
+        RMH_TRY(prof_begin(st, tile == Tile::T128x128 ? PROF_TRAILING : PROF_TRAILING_SMALL, flops, &slot));
         RMH_TRY(launch_trailing<T>(h, g, tile, st));
         return h->prof_end(st, slot);
     };
@@ -405,6 +417,12 @@ rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
         }
     }
     return launch_scatter_diag<T>(A, rs, cs, n, (int)nbi, dfac, n, info, s);
+}
+
+template <typename T>
+rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int64_t* d_info, hipStream_t s) {
+    RMH_TRY(h->reserve(potrf_workspace<T>(h, n)));
+    return potrf_core<T>(h, upper, n, A, lda, d_info, h->ws, false, s);
 }
 
 template <typename T>
@@ -440,6 +458,8 @@ rmh_status trsv_impl(Context* h, bool upper, bool trans, bool unit, int64_t n, c
 }
 
 template rmh_status potrf_impl<double>(Context*, bool, int64_t, double*, int64_t, int64_t*, hipStream_t);
+template size_t potrf_workspace<double>(const Context*, int64_t);
+template rmh_status potrf_core<double>(Context*, bool, int64_t, double*, int64_t, int64_t*, void*, bool, hipStream_t);
 template rmh_status potrf_impl<float>(Context*, bool, int64_t, float*, int64_t, int64_t*, hipStream_t);
 template rmh_status potrs_impl<double>(Context*, bool, int64_t, int64_t, const double*, int64_t, double*, int64_t,
                                        hipStream_t);
@@ -650,6 +670,7 @@ rmh_status rmh_create(rmh_handle* handle, int device) {
     Context* c = new Context();
     c->device = device;
     rmh_status st = c->activate();
+    if (st == RMH_STATUS_SUCCESS) st = noalloc_scratch_init(device);
     if (st != RMH_STATUS_SUCCESS) {
         delete c;
         return st;

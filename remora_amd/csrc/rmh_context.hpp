@@ -105,6 +105,16 @@ template <typename T>
 rmh_status trmm_impl(Context* h, bool left, bool upper, bool trans, bool unit, int64_t m, int64_t n, T alpha,
                      const T* A, int64_t lda, T* B, int64_t ldb, hipStream_t s);
 template <typename T>
+size_t potrf_workspace(const Context* h, int64_t n);
+template <typename T>
+rmh_status potrf_core(Context* h, bool upper, int64_t n, T* A, int64_t lda, int64_t* d_info, void* ws, bool plain,
+                      hipStream_t s);
+// The static device scratch of the entry points that allocate nothing:
+// noalloc_scratch_init resolves its address on `device` (which must be
+// current) once; noalloc_scratch returns it (nullptr before the init).
+rmh_status noalloc_scratch_init(int device);
+void* noalloc_scratch(int device, size_t* bytes);
+template <typename T>
 rmh_status potrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int64_t* d_info, hipStream_t s);
 template <typename T>
 rmh_status potrs_impl(Context* h, bool upper, int64_t n, int64_t nrhs, const T* A, int64_t lda, T* B, int64_t ldb,
