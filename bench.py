@@ -413,25 +413,46 @@ def roofline(tr, dtype, kernel):
 
 
 def cpu_baseline(n=8192, nrhs=64, reps=3):
-    """Oracle (Remora default CPU kernels restated) on one core."""
+    """Oracle (Remora's default CPU kernels restated) on one core, pinned to
+    one CPU like the reference's single-threaded kernels: potrf + potrs of the
+    Gram matrix G G^T / n + 0.1 I with 64 RHS at n = 8192 (the value, about
+    18 s) and n = 4096, and the CPU configuration itself (n = 512, G G^T / 512
+    + 0.1 I, 16 RHS, minimum of 20 runs as examples/Benchmarks/potrf.cpp
+    takes)."""
     from oracle import oracle as orc
     from remora_amd import synth
-    A = synth.gram_spd(n, seed=0)
-    B = synth.rhs(n, nrhs, seed=0)
-    best = float("inf")
-    eta = None
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        X, L, info, _ = orc.posv(A, B)
-        dt = time.perf_counter() - t0
-        best = min(best, dt)
-        assert info == 0
-        eta = synth.backward_error(A, X, B)
-    work = n ** 3 / 3 + 2 * n * n * nrhs
-    return {"value": work / best / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
-            "sample": f"oracle potrf+potrs n={n} nrhs={nrhs} (Gram SPD), min of {reps} runs, 1 thread; "
-                      f"backward error {eta:.2e}; host has {os.cpu_count()} logical CPUs",
-            "seconds": best}
+    # inputs first (their numpy products use every core), then one pinned core
+    cases = [(n, nrhs, reps, None), (4096, 64, 3, None), (512, 16, 20, 512)]
+    inputs = [(synth.gram_spd(m, seed=0, divisor=d), synth.rhs(m, k, seed=0)) for m, k, _, d in cases]
+    try:
+        old_aff = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {min(old_aff)})
+    except (AttributeError, OSError):
+        old_aff = None
+
+    def run(A, B, reps):
+        m, k = B.shape
+        best, eta = float("inf"), None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            X, L, info, _ = orc.posv(A, B)
+            best = min(best, time.perf_counter() - t0)
+            assert info == 0
+            eta = synth.backward_error(A, X, B)
+        work = m ** 3 / 3 + 2 * m * m * k
+        return {"n": m, "nrhs": k, "reps": reps, "seconds": best, "tflops": work / best / 1e12,
+                "backward_error": eta}
+
+    try:
+        res = [run(A, B, c[2]) for (A, B), c in zip(inputs, cases)]
+    finally:
+        if old_aff is not None:
+            os.sched_setaffinity(0, old_aff)
+    main, sizes = res[0], res[1:]
+    return {"value": main["tflops"], "unit": "TFLOP/s", "cores": 1, "kind": "port",
+            "sample": f"oracle potrf+potrs n={n} nrhs={nrhs} (Gram SPD), min of {reps} runs, one thread pinned to "
+                      f"one CPU; backward error {main['backward_error']:.2e}; host has {os.cpu_count()} logical CPUs",
+            "seconds": main["seconds"], "sizes": sizes}
 
 
 def main():

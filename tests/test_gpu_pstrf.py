@@ -137,3 +137,23 @@ def test_laswp_matches_transpositions(inverse, row_major):
     torch.cuda.synchronize()
     got = Bd.cpu().numpy() if row_major else Bd.cpu().numpy().T
     assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("upper", [False, True])
+def test_pstrf_pivot_values_in_global_memory(upper):
+    """Above n = 8192 in FP64 the running diagonal values no longer fit the
+    64 KB of LDS the block kernel gives them and live in global memory: at
+    n = 8200 and rank 300 (A = G G^T with G 8200 x 300, well-separated pivots)
+    the rank and the pivot sequence equal the oracle's, and the factor agrees
+    to 1e-12 of its largest entry."""
+    n, r = 8200, 300
+    print(f"pstrf global pivots n={n} rank={r} upper={upper}", flush=True)
+    rng = np.random.default_rng(77)
+    G = rng.standard_normal((n, r))
+    A = G @ G.T
+    F, P, rank, _ = run(A, upper, ld_pad=0)
+    Fo, Po, ro = orc.pstrf(A, upper=upper, row_major=False)
+    assert rank == ro == r
+    assert np.array_equal(P, Po), "pivot sequence differs from the oracle"
+    L, Lo = factor_of(F, upper)[:, :r], factor_of(Fo, upper)[:, :r]
+    assert np.max(np.abs(L - Lo)) <= 1e-12 * np.max(np.abs(Lo))
