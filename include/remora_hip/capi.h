@@ -312,7 +312,10 @@ rmh_status rmh_set_spotrf_precision(rmh_handle h, rmh_factor_precision prec);
  * panel k's trailing update; the call still completes in the caller's stream
  * order.  Mode 2 also limits the trailing update to one workgroup per compute
  * unit so that the panel kernel's workgroups run beside it; mode 0 is a
- * single stream. */
+ * single stream.  Modes 3 and 4 are modes 2 and 1 with the panel kernel on
+ * four wavefronts per workgroup instead of eight (small enough to share a
+ * compute unit with one trailing-update workgroup); their factors are
+ * identical to each other and differ from modes 0-2 in rounding only. */
 rmh_status rmh_set_potrf_lookahead(rmh_handle h, int enable);
 
 /* ---------------------------------------------------------- profiling -- */

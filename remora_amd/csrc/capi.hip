@@ -313,10 +313,10 @@ rmh_status trsm_impl(Context* h, bool left, bool upper, bool trans, bool unit, i
 // Trailing update of potrf: the MFMA GEMM in T, or for FP32 factors the
 // split-BF16x3 kernel when the handle asks for it (lower, 128 x 128 tiles).
 template <typename T>
-static rmh_status launch_trailing(Context* h, const GemmDesc<T>& g, Tile tile, hipStream_t st) {
+static rmh_status launch_trailing(Context* h, const GemmDesc<T>& g, Tile tile, const SplitPanel* sp, hipStream_t st) {
     if constexpr (sizeof(T) == 4) {
         if ((h->bf16x3_trailing || h->spotrf_bf16x3) && tile == Tile::T128x128 && g.part == Part::Lower)
-            return launch_gemm_bf16x3(g, st);
+            return sp != nullptr ? launch_gemm_bf16x3_split(g, *sp, st) : launch_gemm_bf16x3(g, st);
     }
     return launch_gemm<T>(g, tile, st);
 }
@@ -344,6 +344,26 @@ rmh_status potrf_core(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
     const int64_t rs = upper ? lda : 1, cs = upper ? 1 : lda;
     auto V = [&](int64_t i, int64_t j) { return A + i * rs + j * cs; };
     const bool lookahead = !plain && h->lookahead && n > 2 * nb;
+    // split-BF16x3 trailing updates (FP32 factors, lower storage): the panel
+    // kernel also writes its solved rows split into BF16 hi + lo, in one of
+    // two buffers by panel parity (look-ahead factors panel k+1 while the
+    // update reads panel k's), laid out in 32-deep blocks of rows:
+    // panel column k of row r at (k / 32) * (32 n) + 32 r + k % 32
+    __bf16* split[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    const int64_t sp_kstride = 32 * n;
+    if constexpr (sizeof(T) == 4) {
+        if (!plain && !upper && (h->bf16x3_trailing || h->spotrf_bf16x3) && nbi <= 32) {
+            const size_t per = (size_t)((nb + 31) / 32) * (size_t)sp_kstride;  // elements per array
+            void* base = nullptr;
+            RMH_TRY(h->reserve_slot(WS_SPLIT, 4 * per * sizeof(__bf16), &base));
+            __bf16* b = static_cast<__bf16*>(base);
+            split[0][0] = b;
+            split[0][1] = b + per;
+            split[1][0] = b + 2 * per;
+            split[1][1] = b + 3 * per;
+        }
+    }
+    auto parity = [&](int64_t k0) { return (int)((k0 / nb) & 1); };
     auto prof_begin = [&](hipStream_t st, int cls, double flops, int* slot) {
         *slot = -1;
         return plain ? RMH_STATUS_SUCCESS : h->prof_begin(st, cls, flops, slot);
@@ -360,7 +380,11 @@ rmh_status potrf_core(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
                                  (double)jb * jb * jb / 3.0 + (double)(rows - jb) * jb * jb;
             int slot;
             RMH_TRY(prof_begin(st, PROF_PANEL, flops, &slot));
-            RMH_TRY(launch_potrf_panel<T>(V(j0, k0), rs, cs, rows, K, (int)jb, info, j0, dfac + j0, n, st));
+            __bf16* sh = split[parity(k0)][0];
+            __bf16* sl = split[parity(k0)][1];
+            RMH_TRY(launch_potrf_panel<T>(V(j0, k0), rs, cs, rows, K, (int)jb, info, j0, dfac + j0, n, st,
+                                          sh != nullptr ? sh + j0 * 32 : nullptr, sl != nullptr ? sl + j0 * 32 : nullptr,
+                                          sp_kstride, h->panel_narrow));
             RMH_TRY(h->prof_end(st, slot));
         }
         return RMH_STATUS_SUCCESS;
@@ -386,7 +410,13 @@ rmh_status potrf_core(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
         g.one_per_cu = lookahead && h->share_cus && tile == Tile::T128x128;
         int slot;
         RMH_TRY(prof_begin(st, tile == Tile::T128x128 ? PROF_TRAILING : PROF_TRAILING_SMALL, flops, &slot));
-        RMH_TRY(launch_trailing<T>(h, g, tile, st));
+        SplitPanel sp;
+        sp.hi = split[parity(k0)][0];
+        sp.lo = split[parity(k0)][1];
+        sp.kstride = sp_kstride;
+        sp.a_row0 = c0;
+        sp.b_row0 = c0;
+        RMH_TRY(launch_trailing<T>(h, g, tile, sp.hi != nullptr ? &sp : nullptr, st));
         return h->prof_end(st, slot);
     };
 
@@ -917,9 +947,10 @@ rmh_status rmh_set_spotrf_precision(rmh_handle h, rmh_factor_precision prec) {
 
 rmh_status rmh_set_potrf_lookahead(rmh_handle h, int enable) {
     if (h == nullptr) return fail(RMH_STATUS_INVALID_HANDLE, "NULL handle");
-    RMH_REQUIRE(enable >= 0 && enable <= 2, "set_potrf_lookahead: mode must be 0, 1 or 2");
+    RMH_REQUIRE(enable >= 0 && enable <= 4, "set_potrf_lookahead: mode must be 0 to 4");
     ctx(h)->lookahead = enable != 0;
-    ctx(h)->share_cus = enable == 2;
+    ctx(h)->share_cus = enable == 2 || enable == 3;
+    ctx(h)->panel_narrow = enable == 3 || enable == 4;
     return RMH_STATUS_SUCCESS;
 }
 

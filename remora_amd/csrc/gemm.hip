@@ -484,7 +484,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void gemm_bf16x3_kernel(KParams<float> p) {
+// SPLIT: the operands come split already (launch_potrf_panel's split
+// output, SplitPanel): the stage loads BF16 hi and lo directly, 16 depths of
+// one row per thread as two 16-byte vectors each, with no conversion.
+template <bool SPLIT>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void gemm_bf16x3_kernel(KParams<float> p,
+                                                                                                SplitPanel sp) {
     constexpr int BM = 128, BN = 128, BK = 32, LDK = BK + 8;  // LDS row: 32 depth values + pad
     constexpr int WM = 64, WN = 64, FM = 4, FN = 4;
     __shared__ __bf16 Ah[2][BM * LDK], Al[2][BM * LDK];
@@ -510,6 +515,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     const int mmax = (int)min((int64_t)BM, M - m0) - 1, nmax = (int)min((int64_t)BN, N - n0) - 1;
     const int lda32 = (int)p.lda, ldb32 = (int)p.ldb;
     float ra[16], rb[16];
+    bf16x8 sa[4], sb[4];  // SPLIT: hi(0..7), hi(8..15), lo(0..7), lo(8..15) of A's and B's row
     int st_kmax = 0;
     // full stages load through buffer descriptors: one fixed lane offset,
     // the depth step in the scalar offset (no per-element address VALU work)
@@ -518,6 +524,21 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     auto load_tiles = [&](int64_t k0) {
         const int kmax = (int)min((int64_t)BK, K - k0) - 1;
         st_kmax = kmax;
+        if constexpr (SPLIT) {
+            // stage origin: k-block k0 / 32, this thread's row and depths sk0..sk0+15
+            const int64_t kb = (k0 >> 5) * sp.kstride + sk0;
+            const int64_t ea = kb + (sp.a_row0 + m0 + min(srow, mmax)) * 32;
+            const int64_t eb = kb + (sp.b_row0 + n0 + min(srow, nmax)) * 32;
+            sa[0] = *reinterpret_cast<const bf16x8*>(sp.hi + ea);
+            sa[1] = *reinterpret_cast<const bf16x8*>(sp.hi + ea + 8);
+            sa[2] = *reinterpret_cast<const bf16x8*>(sp.lo + ea);
+            sa[3] = *reinterpret_cast<const bf16x8*>(sp.lo + ea + 8);
+            sb[0] = *reinterpret_cast<const bf16x8*>(sp.hi + eb);
+            sb[1] = *reinterpret_cast<const bf16x8*>(sp.hi + eb + 8);
+            sb[2] = *reinterpret_cast<const bf16x8*>(sp.lo + eb);
+            sb[3] = *reinterpret_cast<const bf16x8*>(sp.lo + eb + 8);
+            return;
+        }
         if (buf_ok && kmax == BK - 1) {
             const __amdgpu_buffer_rsrc_t ar =
                 __builtin_amdgcn_make_buffer_rsrc((void*)uniform_ptr(p.A + m0 + k0 * p.lda), (short)0, 0x7fffffff, kBufWord3);
@@ -555,8 +576,30 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         *reinterpret_cast<bf16x8*>(lo) = l0;
         *reinterpret_cast<bf16x8*>(lo + 8) = l1;
     };
+    // SPLIT: rows past the operand and depths past K are zeroed here
+    auto masked_store = [&](bf16x8 (&v)[4], bool row_ok, int kmax, __bf16* hi, __bf16* lo) {
+        if (!row_ok || sk0 + 15 > kmax) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const bool k0_ok = row_ok && sk0 + e <= kmax, k1_ok = row_ok && sk0 + 8 + e <= kmax;
+                v[0][e] = k0_ok ? v[0][e] : (__bf16)0.0f;
+                v[1][e] = k1_ok ? v[1][e] : (__bf16)0.0f;
+                v[2][e] = k0_ok ? v[2][e] : (__bf16)0.0f;
+                v[3][e] = k1_ok ? v[3][e] : (__bf16)0.0f;
+            }
+        }
+        *reinterpret_cast<bf16x8*>(hi) = v[0];
+        *reinterpret_cast<bf16x8*>(hi + 8) = v[1];
+        *reinterpret_cast<bf16x8*>(lo) = v[2];
+        *reinterpret_cast<bf16x8*>(lo + 8) = v[3];
+    };
     auto store_tiles = [&](int buf) {
         const int kmax = st_kmax;
+        if constexpr (SPLIT) {
+            masked_store(sa, srow <= mmax, kmax, &Ah[buf][srow * LDK + sk0], &Al[buf][srow * LDK + sk0]);
+            masked_store(sb, srow <= nmax, kmax, &Bh[buf][srow * LDK + sk0], &Bl[buf][srow * LDK + sk0]);
+            return;
+        }
         split_store(ra, srow <= mmax, kmax, &Ah[buf][srow * LDK + sk0], &Al[buf][srow * LDK + sk0]);
         split_store(rb, srow <= nmax, kmax, &Bh[buf][srow * LDK + sk0], &Bl[buf][srow * LDK + sk0]);
     };
@@ -795,7 +838,7 @@ rmh_status launch_symm(const GemmDesc<T>& d, bool upper, hipStream_t stream) {
 template rmh_status launch_symm<double>(const GemmDesc<double>&, bool, hipStream_t);
 template rmh_status launch_symm<float>(const GemmDesc<float>&, bool, hipStream_t);
 
-rmh_status launch_gemm_bf16x3(const GemmDesc<float>& d, hipStream_t stream) {
+static rmh_status launch_bf16x3(const GemmDesc<float>& d, const SplitPanel* sp, hipStream_t stream) {
     if (d.M <= 0 || d.N <= 0) return RMH_STATUS_SUCCESS;
     if (d.part != Part::Lower || d.transA || !d.transB || d.splits > 1 || d.batch > 1 || d.k_from_row || d.M < d.N)
         return fail(RMH_STATUS_INVALID_VALUE, "gemm_bf16x3: only C -= A B^T on a lower trapezoid");
@@ -815,8 +858,22 @@ rmh_status launch_gemm_bf16x3(const GemmDesc<float>& d, hipStream_t stream) {
     p.k_chunk = d.K;
     constexpr size_t kStatic = 4 * 128 * 40 * 2 * 2;  // Ah, Al, Bh, Bl, double-buffered
     const size_t dyn = (d.one_per_cu && kStatic < 96 * 1024) ? 96 * 1024 - kStatic : 0;
-    hipLaunchKernelGGL(gemm_bf16x3_kernel, dim3((unsigned)ntiles), dim3(kThreads), dyn, stream, p);
+    if (sp != nullptr) {
+        if (sp->hi == nullptr || sp->lo == nullptr || sp->kstride < 32 * (sp->a_row0 + d.M) ||
+            sp->kstride < 32 * (sp->b_row0 + d.N) || sp->a_row0 < 0 || sp->b_row0 < 0)
+            return fail(RMH_STATUS_INVALID_VALUE, "gemm_bf16x3: split panel does not cover the operands");
+        hipLaunchKernelGGL(gemm_bf16x3_kernel<true>, dim3((unsigned)ntiles), dim3(kThreads), dyn, stream, p, *sp);
+    } else {
+        hipLaunchKernelGGL(gemm_bf16x3_kernel<false>, dim3((unsigned)ntiles), dim3(kThreads), dyn, stream, p,
+                           SplitPanel{});
+    }
     return check_launch("gemm_bf16x3_kernel");
+}
+
+rmh_status launch_gemm_bf16x3(const GemmDesc<float>& d, hipStream_t stream) { return launch_bf16x3(d, nullptr, stream); }
+
+rmh_status launch_gemm_bf16x3_split(const GemmDesc<float>& d, const SplitPanel& sp, hipStream_t stream) {
+    return launch_bf16x3(d, &sp, stream);
 }
 
 Tile pick_tile(int64_t M, int64_t N, Part part) {

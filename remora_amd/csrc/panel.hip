@@ -43,8 +43,10 @@ namespace {
 // Threads per workgroup: blocks up to 32 wide run eight wavefronts, two per
 // SIMD, the wavefronts w and w+4 taking alternate K chunks of the same rows
 // (one f64 MFMA chain per SIMD leaves the matrix unit half idle); 64-wide
-// blocks need the registers of four.
-constexpr int panel_threads(int nb) { return nb <= 32 ? 512 : 256; }
+// blocks need the registers of four.  The narrow form (four wavefronts, one
+// K group) fits beside a trailing-update workgroup whose registers fill
+// more than half of each SIMD, so it can run while that update runs.
+constexpr int panel_threads(int nb, bool narrow = false) { return nb <= 32 && !narrow ? 512 : 256; }
 constexpr int kRW = 64;  // rows per workgroup
 constexpr int kKc = 16;  // K chunk: four 16x16x4 MFMA steps
 
@@ -57,11 +59,12 @@ __device__ __forceinline__ void lower_block(int d, int& bi, int& bj) {
     bj = d;
 }
 
-template <typename T, int NB>
-__global__ __launch_bounds__(panel_threads(NB)) void potrf_panel_kernel(T* P, int64_t rs, int64_t cs, int64_t rows, int K,
+template <typename T, int NB, int THREADS>
+__global__ __launch_bounds__(THREADS) void potrf_panel_kernel(T* P, int64_t rs, int64_t cs, int64_t rows, int K,
                                                                int kb, int64_t* info, int64_t info_base, T* dfac,
-                                                               int64_t ldf) {
-    constexpr int kThreads = panel_threads(NB);
+                                                               int64_t ldf, __bf16* sph, __bf16* spl,
+                                                               int64_t sp_kstride) {
+    constexpr int kThreads = THREADS;
     constexpr int KG = kThreads / 256;        // wavefronts sharing a row block (K groups)
     constexpr int NCB = NB / 16;              // 16-column blocks
     constexpr int NDB = NCB * (NCB + 1) / 2;  // lower blocks of the diagonal block
@@ -76,6 +79,9 @@ __global__ __launch_bounds__(panel_threads(NB)) void potrf_panel_kernel(T* P, in
     __shared__ __align__(16) double col[4 * NB];  // pivot-column buffers (two per factoring wavefront)
     __shared__ int failed;
 
+    // the panel chain is the factorisation's critical path: where a panel
+    // wavefront shares a SIMD with a trailing-update wavefront, it issues first
+    if constexpr (THREADS == 256 && NB <= 32) __builtin_amdgcn_s_setprio(3);
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wr = wave & 3, kg = wave >> 2;  // row block, K group (chunks kg, kg + KG, ...)
@@ -273,6 +279,29 @@ __global__ __launch_bounds__(panel_threads(NB)) void potrf_panel_kernel(T* P, in
 #pragma unroll
             for (int c = 0; c < NB; ++c)
                 if (c < kb) P[i * rs + (int64_t)(K + c) * cs] = (T)r[c];
+            if constexpr (sizeof(T) == 4) {
+                // the stored FP32 value split into BF16 hi + lo for the
+                // split-BF16x3 trailing update: panel column k of row i at
+                // (k / 32) * sp_kstride + i * 32 + k % 32
+                if (sph != nullptr) {
+#pragma unroll
+                    for (int c = 0; c < NB; ++c) {
+                        if (c < kb) {
+                            float v = (float)r[c];
+                            // split the stored FP32 value: without the empty
+                            // asm the compiler merges the two conversions
+                            // into one rounding of the FP64 value to BF16,
+                            // which differs on ties
+                            __asm__("" : "+v"(v));
+                            const __bf16 h = (__bf16)v;
+                            const int k = K + c;
+                            const int64_t e = (int64_t)(k >> 5) * sp_kstride + i * 32 + (k & 31);
+                            sph[e] = h;
+                            spl[e] = (__bf16)(v - (float)h);
+                        }
+                    }
+                }
+            }
         }
         // the factor of the diagonal block goes to the side buffer: every
         // workgroup of this launch still reads the block's original values
@@ -401,22 +430,33 @@ __global__ __launch_bounds__(256) void scatter_diag_kernel(T* A, int64_t rs, int
 
 template <typename T>
 rmh_status launch_potrf_panel(T* P, int64_t rs, int64_t cs, int64_t rows, int64_t K, int kb, int64_t* info,
-                              int64_t info_base, T* dfac, int64_t ldf, hipStream_t stream) {
+                              int64_t info_base, T* dfac, int64_t ldf, hipStream_t stream, __bf16* sph, __bf16* spl,
+                              int64_t sp_kstride, bool narrow) {
     if (kb <= 0) return RMH_STATUS_SUCCESS;
     if (rows < kb) return fail(RMH_STATUS_INVALID_VALUE, "potrf_panel: fewer rows than columns");
     if (K < 0 || K > (int64_t)1 << 30) return fail(RMH_STATUS_INVALID_VALUE, "potrf_panel: bad panel offset");
     const int64_t below = rows - kb;
     const unsigned grid = (unsigned)std::max<int64_t>(1, (below + kRW - 1) / kRW);
     const int k = (int)K;
-    if (kb <= 16)
-        hipLaunchKernelGGL((potrf_panel_kernel<T, 16>), dim3(grid), dim3(panel_threads(16)), 0, stream, P, rs, cs, rows,
-                           k, kb, info, info_base, dfac, ldf);
+    if (sph != nullptr && kb > 32)
+        return fail(RMH_STATUS_INVALID_VALUE, "potrf_panel: split output needs blocks of at most 32 columns");
+    if (kb <= 16 && narrow)
+        hipLaunchKernelGGL((potrf_panel_kernel<T, 16, panel_threads(16, true)>), dim3(grid),
+                           dim3(panel_threads(16, true)), 0, stream, P, rs, cs, rows, k, kb, info, info_base, dfac,
+                           ldf, sph, spl, sp_kstride);
+    else if (kb <= 16)
+        hipLaunchKernelGGL((potrf_panel_kernel<T, 16, panel_threads(16)>), dim3(grid), dim3(panel_threads(16)), 0,
+                           stream, P, rs, cs, rows, k, kb, info, info_base, dfac, ldf, sph, spl, sp_kstride);
+    else if (kb <= 32 && narrow)
+        hipLaunchKernelGGL((potrf_panel_kernel<T, 32, panel_threads(32, true)>), dim3(grid),
+                           dim3(panel_threads(32, true)), 0, stream, P, rs, cs, rows, k, kb, info, info_base, dfac,
+                           ldf, sph, spl, sp_kstride);
     else if (kb <= 32)
-        hipLaunchKernelGGL((potrf_panel_kernel<T, 32>), dim3(grid), dim3(panel_threads(32)), 0, stream, P, rs, cs, rows,
-                           k, kb, info, info_base, dfac, ldf);
+        hipLaunchKernelGGL((potrf_panel_kernel<T, 32, panel_threads(32)>), dim3(grid), dim3(panel_threads(32)), 0,
+                           stream, P, rs, cs, rows, k, kb, info, info_base, dfac, ldf, sph, spl, sp_kstride);
     else if (kb <= 64)
-        hipLaunchKernelGGL((potrf_panel_kernel<T, 64>), dim3(grid), dim3(panel_threads(64)), 0, stream, P, rs, cs, rows,
-                           k, kb, info, info_base, dfac, ldf);
+        hipLaunchKernelGGL((potrf_panel_kernel<T, 64, panel_threads(64)>), dim3(grid), dim3(panel_threads(64)), 0,
+                           stream, P, rs, cs, rows, k, kb, info, info_base, dfac, ldf, nullptr, nullptr, (int64_t)0);
     else
         return fail(RMH_STATUS_INVALID_VALUE, "potrf_panel: block wider than 64");
     return check_launch("potrf_panel_kernel");
@@ -432,9 +472,9 @@ rmh_status launch_scatter_diag(T* A, int64_t rs, int64_t cs, int64_t n, int nbi,
 }
 
 template rmh_status launch_potrf_panel<double>(double*, int64_t, int64_t, int64_t, int64_t, int, int64_t*, int64_t,
-                                               double*, int64_t, hipStream_t);
+                                               double*, int64_t, hipStream_t, __bf16*, __bf16*, int64_t, bool);
 template rmh_status launch_potrf_panel<float>(float*, int64_t, int64_t, int64_t, int64_t, int, int64_t*, int64_t,
-                                              float*, int64_t, hipStream_t);
+                                              float*, int64_t, hipStream_t, __bf16*, __bf16*, int64_t, bool);
 template rmh_status launch_scatter_diag<double>(double*, int64_t, int64_t, int64_t, int, const double*, int64_t,
                                                 const int64_t*, hipStream_t);
 template rmh_status launch_scatter_diag<float>(float*, int64_t, int64_t, int64_t, int, const float*, int64_t,

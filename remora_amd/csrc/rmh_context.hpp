@@ -19,7 +19,7 @@ struct ProfRec {
 
 // Scratch slots of a handle: each grows on demand and is reused by every
 // call that uses it.  Routines that call other routines use different slots.
-enum WsSlot { WS_MAIN = 0, WS_MIXED = 1, WS_FACTOR = 2, WS_ADJOINT = 3, WS_UPDATE = 4, WS_PSTRF = 5, WS_COUNT = 6 };
+enum WsSlot { WS_MAIN = 0, WS_MIXED = 1, WS_FACTOR = 2, WS_ADJOINT = 3, WS_UPDATE = 4, WS_PSTRF = 5, WS_SPLIT = 6, WS_COUNT = 7 };
 
 struct Context {
     // default panel width by precision: FP64 trailing updates gain from
@@ -38,6 +38,7 @@ struct Context {
     // look-ahead variant: trailing updates take one workgroup per compute
     // unit so that a panel workgroup fits beside it
     bool share_cus = false;
+    bool panel_narrow = false;
     // FP32 factorisations run their 128 x 128 lower trailing updates on the
     // split-BF16x3 kernel (set by rmh_dsposv for RMH_FACTOR_BF16X3)
     bool bf16x3_trailing = false;

@@ -88,6 +88,18 @@ rmh_status launch_symm(const GemmDesc<T>& d, bool upper, hipStream_t stream);
 // product (hi*hi + hi*lo + lo*hi), FP32 accumulation; 128 x 128 tiles.
 rmh_status launch_gemm_bf16x3(const GemmDesc<float>& d, hipStream_t stream);
 
+// The same product with the operands already split (launch_potrf_panel's
+// split output): d.A / d.B are not read; A's row r (0 <= r < M) is row
+// a_row0 + r of the split panel, B's row r is row b_row0 + r, and depth k of
+// row r is at (k / 32) * kstride + r * 32 + k % 32 of hi and lo.
+struct SplitPanel {
+    const __bf16* hi = nullptr;
+    const __bf16* lo = nullptr;
+    int64_t kstride = 0;
+    int64_t a_row0 = 0, b_row0 = 0;
+};
+rmh_status launch_gemm_bf16x3_split(const GemmDesc<float>& d, const SplitPanel& sp, hipStream_t stream);
+
 // Pick a tile shape for a problem of this size (fills 256 CUs).
 Tile pick_tile(int64_t M, int64_t N, Part part);
 
@@ -100,9 +112,16 @@ Tile pick_tile(int64_t M, int64_t N, Part part);
 // values) and solves the rows below in place.  On a non-positive pivot j:
 // *info = info_base + j + 1 (the first failure is kept).  Computes in FP64
 // whatever T is.
+// With sph/spl (FP32 factors, kb <= 32) the solved rows are also written
+// split into BF16 hi + lo for the split-BF16x3 trailing update: view element
+// (i, k) at (k / 32) * sp_kstride + i * 32 + k % 32 of sph and spl.  With
+// `narrow`, blocks up to 32 wide run four wavefronts per workgroup instead of
+// eight (same results), small enough to sit beside a trailing-update
+// workgroup on one CU.
 template <typename T>
 rmh_status launch_potrf_panel(T* P, int64_t rs, int64_t cs, int64_t rows, int64_t K, int kb, int64_t* info,
-                              int64_t info_base, T* dfac, int64_t ldf, hipStream_t stream);
+                              int64_t info_base, T* dfac, int64_t ldf, hipStream_t stream, __bf16* sph = nullptr,
+                              __bf16* spl = nullptr, int64_t sp_kstride = 0, bool narrow = false);
 
 // Copy the diagonal-block factors written by launch_potrf_panel (blocks of
 // nbi rows; block j0's element (i, c) at dfac[j0 + i + c*ldf]) into V of

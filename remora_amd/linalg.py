@@ -270,7 +270,8 @@ def set_spotrf_precision(h, prec="fp32"):
 
 def set_potrf_lookahead(h, enable=True):
     """enable: False / True, or the mode 0 (off), 1 (on), 2 (on, trailing
-    update one workgroup per compute unit)."""
+    update one workgroup per compute unit), 3 / 4 (modes 2 / 1 with
+    four-wavefront panel workgroups)."""
     capi.check(capi.load().rmh_set_potrf_lookahead(h.value, int(enable)))
 
 

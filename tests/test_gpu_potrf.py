@@ -189,6 +189,30 @@ def test_potrf_lookahead_is_bitwise_identical(handle, upper):
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
 
 
+@pytest.mark.parametrize("fp32", [False, True])
+def test_potrf_narrow_panels(handle, fp32):
+    """Modes 3 and 4 run the panel on four wavefronts (one K group): the two
+    schedules give the same factor bit for bit, and it is as accurate as the
+    eight-wavefront factor."""
+    from remora_amd import linalg as la
+    n = 4500
+    A = synth.gram_spd(n, seed=12)
+    outs = []
+    for mode in (3, 4, 1):
+        la.set_potrf_lookahead(handle, mode)
+        try:
+            dA = dev(A.astype(np.float32) if fp32 else A)
+            info = la.potrf(handle, dA)
+            outs.append(np.tril(host(dA)).astype(np.float64))
+            assert int(info.item()) == 0
+        finally:
+            la.set_potrf_lookahead(handle, True)
+    assert np.array_equal(outs[0], outs[1])
+    tol = (4 * n * np.finfo(np.float32).eps) if fp32 else (4 * n * np.finfo(float).eps)
+    e_narrow, e_wide = synth.factor_error(A, outs[0]), synth.factor_error(A, outs[2])
+    assert e_narrow < tol and e_narrow < 2 * e_wide + 1e-300, (e_narrow, e_wide)
+
+
 @pytest.mark.parametrize("nbi", [16, 32, 64])
 def test_potrf_large_lookahead_factor_error(handle, nbi):
     """Large n with look-ahead: panel kernels run beside the trailing update,
