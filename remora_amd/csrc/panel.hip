@@ -60,10 +60,9 @@ __device__ __forceinline__ void lower_block(int d, int& bi, int& bj) {
 }
 
 template <typename T, int NB, int THREADS>
-__global__ __launch_bounds__(THREADS) void potrf_panel_kernel(T* P, int64_t rs, int64_t cs, int64_t rows, int K,
-                                                               int kb, int64_t* info, int64_t info_base, T* dfac,
-                                                               int64_t ldf, __bf16* sph, __bf16* spl,
-                                                               int64_t sp_kstride) {
+__device__ __forceinline__ void panel_body(T* P, int64_t rs, int64_t cs, int64_t rows, int K, int kb, int64_t* info,
+                                           int64_t info_base, T* dfac, int64_t ldf, __bf16* sph, __bf16* spl,
+                                           int64_t sp_kstride) {
     constexpr int kThreads = THREADS;
     constexpr int KG = kThreads / 256;        // wavefronts sharing a row block (K groups)
     constexpr int NCB = NB / 16;              // 16-column blocks
@@ -409,6 +408,24 @@ __global__ __launch_bounds__(THREADS) void potrf_panel_kernel(T* P, int64_t rs, 
     }
 }
 
+template <typename T, int NB, int THREADS>
+__global__ __launch_bounds__(THREADS) void potrf_panel_kernel(T* P, int64_t rs, int64_t cs, int64_t rows, int K,
+                                                               int kb, int64_t* info, int64_t info_base, T* dfac,
+                                                               int64_t ldf, __bf16* sph, __bf16* spl,
+                                                               int64_t sp_kstride) {
+    panel_body<T, NB, THREADS>(P, rs, cs, rows, K, kb, info, info_base, dfac, ldf, sph, spl, sp_kstride);
+}
+
+// The narrow form held to 168 registers (three wavefronts' worth per SIMD),
+// so that one workgroup fits in what one trailing-update workgroup leaves
+// when it retires.
+template <typename T, int NB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void potrf_panel_kernel_narrow(
+    T* P, int64_t rs, int64_t cs, int64_t rows, int K, int kb, int64_t* info, int64_t info_base, T* dfac, int64_t ldf,
+    __bf16* sph, __bf16* spl, int64_t sp_kstride) {
+    panel_body<T, NB, 256>(P, rs, cs, rows, K, kb, info, info_base, dfac, ldf, sph, spl, sp_kstride);
+}
+
 }  // namespace
 
 // Copy the diagonal-block factors from the side buffer into the matrix:
@@ -441,16 +458,14 @@ rmh_status launch_potrf_panel(T* P, int64_t rs, int64_t cs, int64_t rows, int64_
     if (sph != nullptr && kb > 32)
         return fail(RMH_STATUS_INVALID_VALUE, "potrf_panel: split output needs blocks of at most 32 columns");
     if (kb <= 16 && narrow)
-        hipLaunchKernelGGL((potrf_panel_kernel<T, 16, panel_threads(16, true)>), dim3(grid),
-                           dim3(panel_threads(16, true)), 0, stream, P, rs, cs, rows, k, kb, info, info_base, dfac,
-                           ldf, sph, spl, sp_kstride);
+        hipLaunchKernelGGL((potrf_panel_kernel_narrow<T, 16>), dim3(grid), dim3(panel_threads(16, true)), 0, stream, P,
+                           rs, cs, rows, k, kb, info, info_base, dfac, ldf, sph, spl, sp_kstride);
     else if (kb <= 16)
         hipLaunchKernelGGL((potrf_panel_kernel<T, 16, panel_threads(16)>), dim3(grid), dim3(panel_threads(16)), 0,
                            stream, P, rs, cs, rows, k, kb, info, info_base, dfac, ldf, sph, spl, sp_kstride);
     else if (kb <= 32 && narrow)
-        hipLaunchKernelGGL((potrf_panel_kernel<T, 32, panel_threads(32, true)>), dim3(grid),
-                           dim3(panel_threads(32, true)), 0, stream, P, rs, cs, rows, k, kb, info, info_base, dfac,
-                           ldf, sph, spl, sp_kstride);
+        hipLaunchKernelGGL((potrf_panel_kernel_narrow<T, 32>), dim3(grid), dim3(panel_threads(32, true)), 0, stream, P,
+                           rs, cs, rows, k, kb, info, info_base, dfac, ldf, sph, spl, sp_kstride);
     else if (kb <= 32)
         hipLaunchKernelGGL((potrf_panel_kernel<T, 32, panel_threads(32)>), dim3(grid), dim3(panel_threads(32)), 0,
                            stream, P, rs, cs, rows, k, kb, info, info_base, dfac, ldf, sph, spl, sp_kstride);

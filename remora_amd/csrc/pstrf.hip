@@ -95,27 +95,26 @@ __global__ __launch_bounds__(kPsThreads) void pstrf_block_kernel(PsView<T> L, in
     __shared__ T wv[kPsThreads / 64];
     __shared__ int wi[kPsThreads / 64];
     __shared__ int s_pivot;
+    __shared__ T rowc[kPsBlock];  // the pivot row in the block's earlier columns
     if (*done != 0) return;
     T* piv = lds_piv ? reinterpret_cast<T*>(ps_dyn) : gpiv;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const double eps = *eps_p;
     const int cur = (int)min((int64_t)kPsBlock, n - k);
+    // this thread's candidate for the next column's pivot, from the running
+    // diagonal values it updated with the column it just computed
+    T nv = T(0);
+    int ni = -1;
     for (int j = 0; j < cur; ++j) {
         const int64_t c = k + j;
-        // running diagonal values (restarted at the block's first column),
-        // updated and scanned for the first largest in one pass
-        T bv = T(0);
-        int bi = -1;
+        // running diagonal values: restarted at the block's first column,
+        // later columns' values were updated (and their candidates found)
+        // where the previous column was computed
+        T bv = nv;
+        int bi = ni;
         if (j == 0) {
             for (int64_t r = k + tid; r < n; r += kPsThreads) {
                 const T x = L(r, r);
-                piv[r] = x;
-                argmax_combine(bv, bi, x, (int)r);
-            }
-        } else {
-            for (int64_t r = c + tid; r < n; r += kPsThreads) {
-                const T y = L(r, c - 1);
-                const T x = piv[r] - y * y;
                 piv[r] = x;
                 argmax_combine(bv, bi, x, (int)r);
             }
@@ -157,9 +156,10 @@ __global__ __launch_bounds__(kPsThreads) void pstrf_block_kernel(PsView<T> L, in
             // rows c and p in the block's own columns; the earlier blocks'
             // columns take the block's swaps afterwards (pstrf_row_swaps_kernel)
             for (int64_t t = k + tid; t < c; t += kPsThreads) {
-                const T x = L(c, t);
-                L(c, t) = L(p, t);
+                const T x = L(c, t), y = L(p, t);
+                L(c, t) = y;
                 L(p, t) = x;
+                rowc[t - k] = y;
             }
             for (int64_t t = c + 1 + tid; t < p; t += kPsThreads) {
                 const T x = L(t, c);
@@ -171,8 +171,10 @@ __global__ __launch_bounds__(kPsThreads) void pstrf_block_kernel(PsView<T> L, in
                 L(t, c) = L(t, p);
                 L(t, p) = x;
             }
-            __syncthreads();
+        } else if (tid < j) {
+            rowc[tid] = L(c, k + tid);
         }
+        __syncthreads();
         const T pv = piv[c];
         if ((double)pv < eps) {
             if (tid == 0) {
@@ -183,11 +185,18 @@ __global__ __launch_bounds__(kPsThreads) void pstrf_block_kernel(PsView<T> L, in
         }
         const T d = sqrt(pv);
         if (tid == 0) L(c, c) = d;
-        // the block's earlier columns have not updated this one yet
+        // the block's earlier columns have not updated this one yet; the
+        // running diagonal value of each row takes the new entry at once
+        nv = T(0);
+        ni = -1;
         for (int64_t r = c + 1 + tid; r < n; r += kPsThreads) {
             T acc = T(0);
-            for (int t = 0; t < j; ++t) acc += L(r, k + t) * L(c, k + t);
-            L(r, c) = (L(r, c) - acc) / d;
+            for (int t = 0; t < j; ++t) acc += L(r, k + t) * rowc[t];
+            const T y = (L(r, c) - acc) / d;
+            L(r, c) = y;
+            const T x = piv[r] - y * y;
+            piv[r] = x;
+            argmax_combine(nv, ni, x, (int)r);
         }
         __syncthreads();
     }
