@@ -279,25 +279,31 @@ __device__ __forceinline__ void panel_body(T* P, int64_t rs, int64_t cs, int64_t
             for (int c = 0; c < NB; ++c)
                 if (c < kb) P[i * rs + (int64_t)(K + c) * cs] = (T)r[c];
             if constexpr (sizeof(T) == 4) {
-                // the stored FP32 value split into BF16 hi + lo for the
+                // the stored FP32 values split into BF16 hi + lo for the
                 // split-BF16x3 trailing update: panel column k of row i at
-                // (k / 32) * sp_kstride + i * 32 + k % 32
+                // (k / 32) * sp_kstride + i * 32 + k % 32, written 8 at a
+                // time (K is a multiple of NB, so the NB columns are one
+                // contiguous run; padded columns hold zeros)
                 if (sph != nullptr) {
+                    typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+                    const int64_t e0 = (int64_t)(K >> 5) * sp_kstride + i * 32 + (K & 31);
 #pragma unroll
-                    for (int c = 0; c < NB; ++c) {
-                        if (c < kb) {
-                            float v = (float)r[c];
+                    for (int c8 = 0; c8 < NB; c8 += 8) {
+                        bf16x8 hv, lv;
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) {
+                            float v = (float)r[c8 + q];
                             // split the stored FP32 value: without the empty
                             // asm the compiler merges the two conversions
                             // into one rounding of the FP64 value to BF16,
                             // which differs on ties
                             __asm__("" : "+v"(v));
                             const __bf16 h = (__bf16)v;
-                            const int k = K + c;
-                            const int64_t e = (int64_t)(k >> 5) * sp_kstride + i * 32 + (k & 31);
-                            sph[e] = h;
-                            spl[e] = (__bf16)(v - (float)h);
+                            hv[q] = h;
+                            lv[q] = (__bf16)(v - (float)h);
                         }
+                        *reinterpret_cast<bf16x8*>(sph + e0 + c8) = hv;
+                        *reinterpret_cast<bf16x8*>(spl + e0 + c8) = lv;
                     }
                 }
             }
