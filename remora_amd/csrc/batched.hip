@@ -139,7 +139,7 @@ __device__ __forceinline__ int64_t factor_one_impl(T* A, int n, int64_t lda, T* 
                 T c4[4];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int i = 16 * t + M::row(lane, r), j = li;
+                    const int i = 16 * t + li, j = M::row(lane, r);
                     c4[r] = (i < rows && j < kb && i >= j) ? Ak[i + (int64_t)j * lda] : T(0);
                 }
                 acc_t acc = {T(0), T(0), T(0), T(0)};
@@ -147,11 +147,11 @@ __device__ __forceinline__ int64_t factor_one_impl(T* A, int n, int64_t lda, T* 
                 for (int q = 0; q < 4; ++q) {
                     const T a = Pbuf(prv, (16 * t + li) * LD + 4 * q + lk);
                     const T b = Pbuf(prv, li * LD + 4 * q + lk);
-                    acc = M::op(a, b, acc);
+                    acc = M::op(b, a, acc);
                 }
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int i = 16 * t + M::row(lane, r), j = li;
+                    const int i = 16 * t + li, j = M::row(lane, r);
                     if (i < rows && j < kb && i >= j) Ak[i + (int64_t)j * lda] = c4[r] - acc[r];
                 }
             }
@@ -233,6 +233,9 @@ __device__ __forceinline__ int64_t factor_one_impl(T* A, int n, int64_t lda, T* 
             __syncthreads();
             continue;
         }
+        // (the MFMAs take the column tile first, so a lane's items are four
+        // columns of one row and 16 lanes cover 16 consecutive rows of a
+        // column: C moves in 128-byte column runs)
         // trailing update of the lower triangle of A22 (m x m): each wave
         // takes groups of G tiles and issues all their C loads before the
         // MFMAs, so G tiles' worth of loads are in flight at once; with a
@@ -251,7 +254,7 @@ __device__ __forceinline__ int64_t factor_one_impl(T* A, int n, int64_t lda, T* 
                 tri_tile(min(t0 + g, ntiles - 1), ti[g], tj[g]);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int i = 16 * ti[g] + M::row(lane, r), j = 16 * tj[g] + li;
+                    const int i = 16 * ti[g] + li, j = 16 * tj[g] + M::row(lane, r);
                     c[g][r] = (t0 + g < ntiles && i < m && j < m && i >= j) ? A22[i + (int64_t)j * lda] : T(0);
                 }
             }
@@ -263,19 +266,19 @@ __device__ __forceinline__ int64_t factor_one_impl(T* A, int n, int64_t lda, T* 
                 for (int q = 0; q < NB / 4; ++q) {
                     const T a = Pbuf(cur, (16 * ti[g] + li) * LD + 4 * q + lk);
                     const T b = Pbuf(cur, (16 * tj[g] + li) * LD + 4 * q + lk);
-                    acc = M::op(a, b, acc);
+                    acc = M::op(b, a, acc);
                 }
                 if (two) {
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const T a = Pbuf(prv, (16 + 16 * ti[g] + li) * LD + 4 * q + lk);
                         const T b = Pbuf(prv, (16 + 16 * tj[g] + li) * LD + 4 * q + lk);
-                        acc = M::op(a, b, acc);
+                        acc = M::op(b, a, acc);
                     }
                 }
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int i = 16 * ti[g] + M::row(lane, r), j = 16 * tj[g] + li;
+                    const int i = 16 * ti[g] + li, j = 16 * tj[g] + M::row(lane, r);
                     if (i < m && j < m && i >= j) A22[i + (int64_t)j * lda] = c[g][r] - acc[r];
                 }
             }
