@@ -26,6 +26,7 @@
 // factorisation and the small solves are FP64; the trailing update is on
 // MFMA in T.
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "rmh_context.hpp"
@@ -328,27 +329,43 @@ __device__ __forceinline__ void solve_one_impl(const T* A, int n, int64_t lda, T
                                                bool forward_done) {
     Smem<T, NB>& s = g_bsm<T, NB>;
     const int tid = threadIdx.x;
-    constexpr int LD = Smem<T, NB>::LD, XL = kMaxRhs + 1;
-    auto load_diag = [&](int k, int kb) {
-        for (int e = tid; e < NB * NB; e += kThreads) {
-            const int i = e % NB, c = e / NB;
-            s.L[i * LD + c] = (i < kb && c <= i) ? (double)A[(k + i) + (int64_t)(k + c) * lda] : (i == c ? 1.0 : 0.0);
+    constexpr int XL = kMaxRhs + 1;
+    // the right-hand sides stay in LDS for both sweeps, in the P buffer the
+    // factorisation no longer needs (n x (nrhs + 1) <= kMaxN x (NB + 1)
+    // elements), with the same rounding to T after every update as in
+    // global memory
+    T* Y = s.P;
+    const int YL = nrhs + 1;
+    for (int e = tid; e < n * nrhs; e += kThreads) {
+        const int i = e % n, j = e / n;
+        Y[i * YL + j] = B[i + (int64_t)j * ldb];
+    }
+    __syncthreads();
+    // diagonal block of BS rows at k (identity-padded) into Ld (ld BS + 1),
+    // its right-hand sides into Xd
+    auto load_diag = [&](auto bs_c, int k, int kb, double* Ld, double* Xd) {
+        constexpr int BS = decltype(bs_c)::value, LDB = BS + 1;
+        for (int e = tid; e < BS * BS; e += kThreads) {
+            const int i = e % BS, c = e / BS;
+            Ld[i * LDB + c] = (i < kb && c <= i) ? (double)A[(k + i) + (int64_t)(k + c) * lda] : (i == c ? 1.0 : 0.0);
         }
-        for (int e = tid; e < NB * nrhs; e += kThreads) {
-            const int i = e % NB, j = e / NB;
-            s.X[i * XL + j] = (i < kb) ? (double)B[(k + i) + (int64_t)j * ldb] : 0.0;
+        for (int e = tid; e < BS * nrhs; e += kThreads) {
+            const int i = e % BS, j = e / BS;
+            Xd[i * XL + j] = (i < kb) ? (double)Y[(k + i) * YL + j] : 0.0;
         }
     };
-    // forward: L y = b
+    // forward: L y = b, in the factorisation's NB blocks (the fused
+    // forward substitution of factor_one_impl gives the same Y bit for bit)
     for (int k = 0; k < n && !forward_done; k += NB) {
+        constexpr int LD = NB + 1;
         const int kb = min(NB, n - k), m = n - k - kb;
-        load_diag(k, kb);
+        load_diag(std::integral_constant<int, NB>{}, k, kb, s.L, s.X);
         __syncthreads();
         diag_solve<NB, LD, XL>(s.L, s.X, nrhs, true, tid);  // L_kk y = b
         __syncthreads();
         for (int e = tid; e < kb * nrhs; e += kThreads) {
             const int i = e % kb, j = e / kb;
-            B[(k + i) + (int64_t)j * ldb] = (T)s.X[i * XL + j];
+            Y[(k + i) * YL + j] = (T)s.X[i * XL + j];
         }
         for (int r = tid; r < m; r += kThreads) {  // b_r -= L[r, blk] y_blk
             const T* lrow = A + (k + kb + r) + (int64_t)k * lda;
@@ -359,39 +376,47 @@ __device__ __forceinline__ void solve_one_impl(const T* A, int n, int64_t lda, T
                 double acc = 0.0;
 #pragma unroll
                 for (int c = 0; c < NB; ++c) acc = fma(l[c], s.X[c * XL + j], acc);
-                T* b = B + (k + kb + r) + (int64_t)j * ldb;
+                T* b = &Y[(k + kb + r) * YL + j];
                 *b = (T)((double)*b - acc);
             }
         }
         __syncthreads();
     }
     // backward: L^T x = y, last block first
-    const int nblk = (n + NB - 1) / NB;
+    constexpr int SB = NB, SLD = SB + 1;
+    double* const SL = s.L;
+    double* const SX = s.X;
+    const int nblk = (n + SB - 1) / SB;
     for (int q = nblk - 1; q >= 0; --q) {
-        const int k = q * NB, kb = min(NB, n - k);
-        load_diag(k, kb);
+        const int k = q * SB, kb = min(SB, n - k);
+        load_diag(std::integral_constant<int, SB>{}, k, kb, SL, SX);
         __syncthreads();
-        diag_solve<NB, LD, XL>(s.L, s.X, nrhs, false, tid);  // L_kk^T x = y
+        diag_solve<SB, SLD, XL>(SL, SX, nrhs, false, tid);  // L_kk^T x = y
         __syncthreads();
         for (int e = tid; e < kb * nrhs; e += kThreads) {
             const int i = e % kb, j = e / kb;
-            B[(k + i) + (int64_t)j * ldb] = (T)s.X[i * XL + j];
+            Y[(k + i) * YL + j] = (T)SX[i * XL + j];
         }
         for (int r = tid; r < k; r += kThreads) {  // y_r -= L[blk, r]^T x_blk
             const T* lcol = A + k + (int64_t)r * lda;  // L(k + c, r), contiguous in c
-            double l[NB];
+            double l[SB];
 #pragma unroll
-            for (int c = 0; c < NB; ++c) l[c] = (c < kb) ? (double)lcol[c] : 0.0;
+            for (int c = 0; c < SB; ++c) l[c] = (c < kb) ? (double)lcol[c] : 0.0;
             for (int j = 0; j < nrhs; ++j) {
                 double acc = 0.0;
 #pragma unroll
-                for (int c = 0; c < NB; ++c) acc = fma(l[c], s.X[c * XL + j], acc);
-                T* b = B + r + (int64_t)j * ldb;
+                for (int c = 0; c < SB; ++c) acc = fma(l[c], SX[c * XL + j], acc);
+                T* b = &Y[r * YL + j];
                 *b = (T)((double)*b - acc);
             }
         }
         __syncthreads();
     }
+    for (int e = tid; e < n * nrhs; e += kThreads) {
+        const int i = e % n, j = e / n;
+        B[i + (int64_t)j * ldb] = Y[i * YL + j];
+    }
+    __syncthreads();
 }
 
 // Out-of-line copies: each routine then gets its own register allocation
