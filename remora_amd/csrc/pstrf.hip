@@ -203,19 +203,53 @@ __global__ __launch_bounds__(kPsThreads) void pstrf_block_kernel(PsView<T> L, in
 }
 
 // the row swaps of block k .. k+cur-1, in order, on the columns before k (not
-// read while the block runs): one thread per column
+// read while the block runs).  The block's transpositions are first composed
+// into one permutation of the (at most 2 * 20) rows they touch -- wavefront
+// 0 keeps the list one row per lane -- and then thread (q, y) of a workgroup
+// moves entry q of column 4 * blockIdx.x + y: all loads of a column before a
+// barrier, all stores after it, instead of one dependent swap after another.
+constexpr int kSwapCols = 4;  // columns per workgroup (64 x 4 threads)
 template <typename T>
-__global__ __launch_bounds__(256) void pstrf_row_swaps_kernel(PsView<T> L, int64_t k, int cur,
-                                                             const int64_t* __restrict__ P) {
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (t >= k) return;
-    for (int j = 0; j < cur; ++j) {
-        const int64_t c = k + j, p = P[c];
-        if (p == c) continue;
-        const T x = L(c, t);
-        L(c, t) = L(p, t);
-        L(p, t) = x;
+__global__ __launch_bounds__(64 * kSwapCols) void pstrf_row_swaps_kernel(PsView<T> L, int64_t k, int cur,
+                                                                        const int64_t* __restrict__ P) {
+    constexpr int R = 2 * kPsBlock;
+    __shared__ int64_t rows[R];  // the touched rows
+    __shared__ int64_t src[R];   // the row whose entry ends up in rows[q]
+    __shared__ int nrows;
+    const int lane = threadIdx.x, y = threadIdx.y;
+    if (y == 0) {
+        int64_t myrow = -1;  // list slot `lane`: a touched row and where its entry comes from
+        int myfrom = lane;
+        int cnt = 0;
+        for (int j = 0; j < cur; ++j) {
+            const int64_t c = k + j, p = P[c];
+            if (p == c) continue;
+            unsigned long long m = __ballot(myrow == c);
+            int a = cnt;
+            if (m) a = __ffsll((long long)m) - 1;
+            else { if (lane == cnt) myrow = c; ++cnt; }
+            m = __ballot(myrow == p);
+            int b = cnt;
+            if (m) b = __ffsll((long long)m) - 1;
+            else { if (lane == cnt) myrow = p; ++cnt; }
+            const int fa = __shfl(myfrom, a), fb = __shfl(myfrom, b);
+            if (lane == a) myfrom = fb;
+            if (lane == b) myfrom = fa;
+        }
+        const int64_t from_row = __shfl(myrow, myfrom);
+        if (lane < R) {
+            rows[lane] = myrow;
+            src[lane] = from_row;
+        }
+        if (lane == 0) nrows = cnt;
     }
+    __syncthreads();
+    const int64_t t = (int64_t)blockIdx.x * kSwapCols + y;
+    const bool active = t < k && lane < nrows;
+    T v = T(0);
+    if (active) v = L(src[lane], t);
+    __syncthreads();
+    if (active) L(rows[lane], t) = v;
 }
 
 // the reference's zeros: the triangle opposite the factor, and after a stop
@@ -284,8 +318,8 @@ rmh_status pstrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
         RMH_TRY(check_launch("pstrf_block_kernel"));
         if (k > 0) {
             const int cur = (int)std::min<int64_t>(kPsBlock, n - k);
-            hipLaunchKernelGGL((pstrf_row_swaps_kernel<T>), dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, L, k,
-                               cur, piv_out);
+            hipLaunchKernelGGL((pstrf_row_swaps_kernel<T>), dim3((unsigned)((k + kSwapCols - 1) / kSwapCols)),
+                               dim3(64, kSwapCols), 0, s, L, k, cur, piv_out);
             RMH_TRY(check_launch("pstrf_row_swaps_kernel"));
         }
         const int64_t m2 = n - k - kPsBlock;
