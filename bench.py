@@ -507,7 +507,7 @@ def main():
         paths["rbf_fp64"] = {"tflops": work / (med64 * 1e-3) / 1e12, "ms": med64, "step_ms": r["step_ms"]}
         acc["rbf_fp64"] = {"backward_error": eta, "factor_error_sampled": ferr}
         roof64 = roofline(r["trailing"], "f64", "gemm_kernel<double,128,128,16,false,true,1,0>: trailing SYRK update, "
-                                                      "lower trapezoid, K = panel width (512 for FP64)")
+                                                      "lower trapezoid, K = panel width (1024 for FP64)")
         log(f"[bench] rbf fp64: median {med64:.3f} ms  {paths['rbf_fp64']['tflops']:.2f} TFLOP/s  eta {eta:.2e}  "
             f"ferr {ferr:.2e}")
         del r
@@ -520,7 +520,7 @@ def main():
                                        "iters": r["iters"]}
             acc["rbf_mixed_fp32"] = {"backward_error": eta, "reported_berr": r["berr"]}
             roof32 = roofline(r["trailing"], "f32", "gemm_kernel<float,128,128,32,false,true,1,0>: trailing SYRK update, "
-                                                      "lower trapezoid, K = panel width (256 for FP32)")
+                                                      "lower trapezoid, K = panel width (1024 for FP32)")
             log(f"[bench] rbf mixed: median {medmx:.3f} ms  {paths['rbf_mixed_fp32']['tflops']:.2f} TFLOP/s  "
                 f"eta {eta:.2e}  iters {r['iters']}")
             assert max(r["iters"]) >= 0 and eta <= TOL, "mixed path did not reach the tolerance"

@@ -300,22 +300,33 @@ rmh_status rmh_batched_autotune(rmh_handle h, int fp64, size_t batch, const int6
 /* ------------------------------------------------------------ tuning -- */
 /* Block sizes of rmh_?potrf: nb is the width of a panel (the K of the
  * trailing update), nb_inner the diagonal block inside a panel (<= 64).
- * 0 restores the defaults (nb 512 for FP64, 256 for FP32; nb_inner 32).
- * Applies to later calls on this handle. */
+ * 0 restores the defaults (nb 1024 for FP64 and FP32, 768 for split-BF16x3
+ * FP32 factors; nb_inner 32).  Applies to later calls on this handle. */
 rmh_status rmh_set_potrf_blocking(rmh_handle h, size_t nb, size_t nb_inner);
+/* Leaves of the recursive panel factorisation of rmh_?potrf: a panel is
+ * halved recursively (factor the left half, update the right half with it on
+ * the MFMA GEMM, factor the right half) down to leaves of at most `leaf`
+ * columns, each factored by a chain of fused block-column launches.  Rounded
+ * down to a multiple of max(nb_inner, 32); 0 restores the default 128; a
+ * leaf at least as wide as the panel factors every panel as one chain.  No
+ * counterpart in the reference (a tuning knob of this library).  Applies to
+ * later calls on this handle. */
+rmh_status rmh_set_potrf_leaf(rmh_handle h, size_t leaf);
 /* Precision of the trailing updates of rmh_spotrf on this handle:
  * RMH_FACTOR_FP32 (default, FP32 MFMA) or RMH_FACTOR_BF16X3 (split-BF16x3,
  * lower storage).  rmh_dsposv sets its own choice for its factorisation. */
 rmh_status rmh_set_spotrf_precision(rmh_handle h, rmh_factor_precision prec);
-/* Look-ahead (mode 1, the default): rmh_?potrf factors panel k+1 on a
- * second, handle-owned stream while the caller's stream applies the rest of
- * panel k's trailing update; the call still completes in the caller's stream
- * order.  Mode 2 also limits the trailing update to one workgroup per compute
- * unit so that the panel kernel's workgroups run beside it; mode 0 is a
- * single stream.  Modes 3 and 4 are modes 2 and 1 with the panel kernel on
- * four wavefronts per workgroup instead of eight (small enough to share a
- * compute unit with one trailing-update workgroup); their factors are
- * identical to each other and differ from modes 0-2 in rounding only. */
+/* Look-ahead mode 1: rmh_?potrf factors panel k+1 on a second,
+ * handle-owned stream while the caller's stream applies the rest of panel
+ * k's trailing update; the call still completes in the caller's stream
+ * order.  Mode 2 also limits the trailing update to one workgroup per
+ * compute unit so that the panel kernel's workgroups run beside it; mode 0
+ * is a single stream.  Modes 3 and 4 are modes 2 and 1 with the panel
+ * kernel on four wavefronts per workgroup instead of eight (small enough to
+ * share a compute unit with one trailing-update workgroup); their factors
+ * are identical to each other and differ from modes 0-2 in rounding only.
+ * -1 (the default) picks by precision: mode 1 for rmh_dpotrf, mode 4 for
+ * rmh_spotrf. */
 rmh_status rmh_set_potrf_lookahead(rmh_handle h, int enable);
 
 /* ---------------------------------------------------------- profiling -- */

@@ -334,8 +334,16 @@ size_t potrf_workspace(const Context* h, int64_t n) {
 template <typename T>
 rmh_status potrf_core(Context* h, bool upper, int64_t n, T* A, int64_t lda, int64_t* d_info, void* ws, bool plain,
                       hipStream_t s) {
-    const int64_t nb = (int64_t)(h->nb != 0 ? h->nb : (sizeof(T) == 8 ? Context::kDefaultNbF64 : Context::kDefaultNbF32));
+    const bool bf16x3 = sizeof(T) == 4 && (h->bf16x3_trailing || h->spotrf_bf16x3);
+    const int64_t nb = (int64_t)(h->nb != 0 ? h->nb
+                                            : (sizeof(T) == 8 ? Context::kDefaultNbF64
+                                                              : (bf16x3 ? Context::kDefaultNbBf16x3 : Context::kDefaultNbF32)));
     const int64_t nbi = (int64_t)h->nbi;
+    // look-ahead mode: 0 single stream; 1 look-ahead; 2 with one trailing
+    // workgroup per compute unit; 3 and 4: modes 2 and 1 with four-wavefront
+    // panel workgroups
+    const int mode = h->la_mode >= 0 ? h->la_mode : (sizeof(T) == 8 ? 1 : 4);
+    const bool share_cus = mode == 2 || mode == 3, panel_narrow = mode == 3 || mode == 4;
     int64_t* scratch = static_cast<int64_t*>(ws);
     int64_t* info = d_info != nullptr ? d_info : scratch;
     T* dfac = reinterpret_cast<T*>(static_cast<char*>(ws) + 256);
@@ -343,7 +351,7 @@ rmh_status potrf_core(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
     if (n == 0) return RMH_STATUS_SUCCESS;
     const int64_t rs = upper ? lda : 1, cs = upper ? 1 : lda;
     auto V = [&](int64_t i, int64_t j) { return A + i * rs + j * cs; };
-    const bool lookahead = !plain && h->lookahead && n > 2 * nb;
+    const bool lookahead = !plain && mode != 0 && n > 2 * nb;
     // split-BF16x3 trailing updates (FP32 factors, lower storage): the panel
     // kernel also writes its solved rows split into BF16 hi + lo, in one of
     // two buffers by panel parity (look-ahead factors panel k+1 while the
@@ -372,22 +380,67 @@ rmh_status potrf_core(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
     // Factor the panel V[k0:, k0:k0+kw] left-looking, one fused kernel per
     // block column [j0, j0+jb): it applies the updates of the panel's earlier
     // columns [k0, j0), factors the diagonal block and solves the rows below.
-    auto factor_panel = [&](int64_t k0, int64_t kw, hipStream_t st) -> rmh_status {
-        for (int64_t j0 = k0; j0 < k0 + kw; j0 += nbi) {
-            const int64_t jb = std::min(nbi, k0 + kw - j0);
-            const int64_t rows = n - j0, K = j0 - k0;
+    // chain of fused panel launches over the block columns of [s0, s0+sw),
+    // each left-looking from s0; the panel starts at p0
+    auto chain = [&](int64_t p0, int64_t s0, int64_t sw, hipStream_t st) -> rmh_status {
+        for (int64_t j0 = s0; j0 < s0 + sw; j0 += nbi) {
+            const int64_t jb = std::min(nbi, s0 + sw - j0);
+            const int64_t rows = n - j0, K = j0 - s0;
             const double flops = 2.0 * K * ((double)jb * (jb + 1) / 2.0 + (double)(rows - jb) * jb) +
                                  (double)jb * jb * jb / 3.0 + (double)(rows - jb) * jb * jb;
             int slot;
             RMH_TRY(prof_begin(st, PROF_PANEL, flops, &slot));
-            __bf16* sh = split[parity(k0)][0];
-            __bf16* sl = split[parity(k0)][1];
-            RMH_TRY(launch_potrf_panel<T>(V(j0, k0), rs, cs, rows, K, (int)jb, info, j0, dfac + j0, n, st,
-                                          sh != nullptr ? sh + j0 * 32 : nullptr, sl != nullptr ? sl + j0 * 32 : nullptr,
-                                          sp_kstride, h->panel_narrow));
+            // split output: panel column (j0 - p0) of the row; the chain's
+            // first column (a multiple of 32 past p0) folded into the base
+            const int64_t soff = ((s0 - p0) >> 5) * sp_kstride + j0 * 32;
+            __bf16* sh = split[parity(p0)][0];
+            __bf16* sl = split[parity(p0)][1];
+            RMH_TRY(launch_potrf_panel<T>(V(j0, s0), rs, cs, rows, K, (int)jb, info, j0, dfac + j0, n, st,
+                                          sh != nullptr ? sh + soff : nullptr, sl != nullptr ? sl + soff : nullptr,
+                                          sp_kstride, panel_narrow));
             RMH_TRY(h->prof_end(st, slot));
         }
         return RMH_STATUS_SUCCESS;
+    };
+    // V[c0:, c0:c1] -= S S[c0:c1]^T, S = V[c0:, s0:s0+sw]: the panel's columns
+    // right of a factored part take its update
+    auto in_panel_update = [&](int64_t s0, int64_t sw, int64_t c0, int64_t c1, hipStream_t st) -> rmh_status {
+        if (c0 >= c1) return RMH_STATUS_SUCCESS;
+        GemmDesc<T> g;
+        g.alpha = T(-1); g.beta = T(1); g.K = sw;
+        g.M = n - c0; g.N = c1 - c0; g.part = Part::Lower;
+        if (!upper) {
+            g.A = A + c0 + s0 * lda; g.lda = lda; g.transA = false;
+            g.B = g.A; g.ldb = lda; g.transB = true;
+        } else {
+            g.M = c1 - c0; g.N = n - c0; g.part = Part::Upper;
+            g.A = A + s0 + c0 * lda; g.lda = lda; g.transA = true;
+            g.B = g.A; g.ldb = lda; g.transB = false;
+        }
+        g.C = A + c0 + c0 * lda; g.ldc = lda;
+        const double flops = 2.0 * sw * ((double)(c1 - c0) * (c1 - c0 + 1) / 2.0 + (double)(n - c1) * (c1 - c0));
+        int slot;
+        RMH_TRY(prof_begin(st, PROF_PANEL, flops, &slot));
+        RMH_TRY(launch_gemm<T>(g, pick_tile(g.M, g.N, g.part), st));
+        return h->prof_end(st, slot);
+    };
+    // recursive halving of [s0, s0+sw) down to leaves of at most `leafw`
+    // columns (a multiple of nbi), each a chain left-looking from its first
+    // column: factor the left half, update the right half with it, factor
+    // the right half.  The chains' K loops stay short (at most leafw - nbi
+    // deep); the depth above a leaf is taken by MFMA GEMMs.
+    const int64_t lq = std::max<int64_t>(nbi, 32);  // split output needs 32-aligned chain starts
+    const int64_t leafw = std::max<int64_t>(lq, (int64_t)h->nb_leaf / lq * lq);
+    auto rec = [&](auto& self, int64_t p0, int64_t s0, int64_t sw, hipStream_t st) -> rmh_status {
+        if (sw <= leafw) return chain(p0, s0, sw, st);
+        const int64_t h1 = ((sw / 2 + leafw - 1) / leafw) * leafw;
+        RMH_TRY(self(self, p0, s0, h1, st));
+        RMH_TRY(in_panel_update(s0, h1, s0 + h1, s0 + sw, st));
+        return self(self, p0, s0 + h1, sw - h1, st);
+    };
+    // Factor the panel V[k0:, k0:k0+kw].
+    auto factor_panel = [&](int64_t k0, int64_t kw, hipStream_t st) -> rmh_status {
+        return rec(rec, k0, k0, kw, st);
     };
     // V[c0:, c0:c1] -= P P[c0:c1]^T on the lower trapezoid, P = V[c0:, k0:k0+kw]
     auto trailing = [&](int64_t k0, int64_t kw, int64_t c0, int64_t c1, hipStream_t st) -> rmh_status {
@@ -407,7 +460,7 @@ rmh_status potrf_core(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
         g.C = A + c0 + c0 * lda; g.ldc = lda;
         const double flops = 2.0 * kw * ((double)cols * (cols + 1) / 2.0 + (double)(rows - cols) * cols);
         const Tile tile = pick_tile(g.M, g.N, g.part);
-        g.one_per_cu = lookahead && h->share_cus && tile == Tile::T128x128;
+        g.one_per_cu = lookahead && share_cus && tile == Tile::T128x128;
         int slot;
         RMH_TRY(prof_begin(st, tile == Tile::T128x128 ? PROF_TRAILING : PROF_TRAILING_SMALL, flops, &slot));
         SplitPanel sp;
@@ -947,10 +1000,8 @@ rmh_status rmh_set_spotrf_precision(rmh_handle h, rmh_factor_precision prec) {
 
 rmh_status rmh_set_potrf_lookahead(rmh_handle h, int enable) {
     if (h == nullptr) return fail(RMH_STATUS_INVALID_HANDLE, "NULL handle");
-    RMH_REQUIRE(enable >= 0 && enable <= 4, "set_potrf_lookahead: mode must be 0 to 4");
-    ctx(h)->lookahead = enable != 0;
-    ctx(h)->share_cus = enable == 2 || enable == 3;
-    ctx(h)->panel_narrow = enable == 3 || enable == 4;
+    RMH_REQUIRE(enable >= -1 && enable <= 4, "set_potrf_lookahead: mode must be -1 to 4");
+    ctx(h)->la_mode = enable;
     return RMH_STATUS_SUCCESS;
 }
 
@@ -963,6 +1014,12 @@ rmh_status rmh_set_potrf_blocking(rmh_handle h, size_t nb, size_t nb_inner) {
                 "set_potrf_blocking: nb (and the default widths) must be multiples of nb_inner");
     ctx(h)->nb = nb;
     ctx(h)->nbi = nb_inner;
+    return RMH_STATUS_SUCCESS;
+}
+
+rmh_status rmh_set_potrf_leaf(rmh_handle h, size_t leaf) {
+    if (h == nullptr) return fail(RMH_STATUS_INVALID_HANDLE, "NULL handle");
+    ctx(h)->nb_leaf = leaf == 0 ? Context::kDefaultNbLeaf : leaf;
     return RMH_STATUS_SUCCESS;
 }
 

@@ -22,10 +22,14 @@ struct ProfRec {
 enum WsSlot { WS_MAIN = 0, WS_MIXED = 1, WS_FACTOR = 2, WS_ADJOINT = 3, WS_UPDATE = 4, WS_PSTRF = 5, WS_SPLIT = 6, WS_COUNT = 7 };
 
 struct Context {
-    // default panel width by precision: FP64 trailing updates gain from
-    // K = 512 (half the C traffic), FP32 ones keep the panel chain short
-    static constexpr size_t kDefaultNbF64 = 512, kDefaultNbF32 = 256;
+    // default panel width by precision (the K of the trailing updates; the
+    // panels are factored recursively, so a wide panel costs little on the
+    // panel chain): 1024 for FP64 and FP32, 768 for split-BF16x3 factors,
+    // whose trailing updates are cheap enough that a panel's GEMMs and
+    // chains balance there (measured at n = 16384)
+    static constexpr size_t kDefaultNbF64 = 1024, kDefaultNbF32 = 1024, kDefaultNbBf16x3 = 768;
     static constexpr size_t kDefaultNbi = 32;
+    static constexpr size_t kDefaultNbLeaf = 128;
 
     int device = 0;
     void* ws = nullptr;  // WS_MAIN (potrf / trsm scratch)
@@ -33,12 +37,12 @@ struct Context {
     void* aux[WS_COUNT] = {};
     size_t aux_bytes[WS_COUNT] = {};
     size_t nb = 0, nbi = kDefaultNbi;  // nb 0: the default for the precision
+    size_t nb_leaf = kDefaultNbLeaf;     // leaf width of the recursive panel factorisation
     int prof_mask = 0;  // bit i: record launches of ProfClass i
-    bool lookahead = true;
-    // look-ahead variant: trailing updates take one workgroup per compute
-    // unit so that a panel workgroup fits beside it
-    bool share_cus = false;
-    bool panel_narrow = false;
+    // look-ahead mode (rmh_set_potrf_lookahead); -1: by precision, mode 1
+    // for FP64 factorisations and mode 4 (four-wavefront panel workgroups)
+    // for FP32 ones (measured fastest at n = 16384)
+    int la_mode = -1;
     // FP32 factorisations run their 128 x 128 lower trailing updates on the
     // split-BF16x3 kernel (set by rmh_dsposv for RMH_FACTOR_BF16X3)
     bool bf16x3_trailing = false;

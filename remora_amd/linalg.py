@@ -263,15 +263,20 @@ def set_potrf_blocking(h, nb=0, nb_inner=0):
     capi.check(capi.load().rmh_set_potrf_blocking(h.value, nb, nb_inner))
 
 
+def set_potrf_leaf(h, leaf=0):
+    """Leaf width of the recursive panel factorisation (0: the default)."""
+    capi.check(capi.load().rmh_set_potrf_leaf(h.value, leaf))
+
+
 def set_spotrf_precision(h, prec="fp32"):
     p = {"fp32": capi.FACTOR_FP32, "bf16x3": capi.FACTOR_BF16X3}[prec]
     capi.check(capi.load().rmh_set_spotrf_precision(h.value, p))
 
 
-def set_potrf_lookahead(h, enable=True):
+def set_potrf_lookahead(h, enable=-1):
     """enable: False / True, or the mode 0 (off), 1 (on), 2 (on, trailing
     update one workgroup per compute unit), 3 / 4 (modes 2 / 1 with
-    four-wavefront panel workgroups)."""
+    four-wavefront panel workgroups); -1 (the default) picks by precision."""
     capi.check(capi.load().rmh_set_potrf_lookahead(h.value, int(enable)))
 
 

@@ -87,6 +87,36 @@ def test_potrf_blocking_variants(handle, nb, nbi):
     assert synth.factor_error(A, L) < 4 * n * np.finfo(float).eps
 
 
+@pytest.mark.parametrize("fp32", [False, True])
+@pytest.mark.parametrize("leaf", [32, 64, 128, 384, 4096])
+def test_potrf_recursive_panel_leaves(handle, leaf, fp32):
+    """Panels are halved recursively down to leaves of `leaf` columns (a
+    leaf as wide as the panel is a single chain): every leaf width gives an
+    n*u-accurate factor, and the look-ahead schedule is bitwise identical to
+    the single stream for each."""
+    import torch
+    from remora_amd import linalg as la
+    n = 3100
+    A = synth.gram_spd(n, seed=17)
+    la.set_potrf_blocking(handle, 1024, 32)
+    la.set_potrf_leaf(handle, leaf)
+    outs = []
+    try:
+        for mode in (1, 0):
+            la.set_potrf_lookahead(handle, mode)
+            dA = dev(A.astype(np.float32) if fp32 else A, torch.float32 if fp32 else torch.float64)
+            info = la.potrf(handle, dA)
+            assert int(info.item()) == 0
+            outs.append(np.tril(host(dA)).astype(np.float64))
+    finally:
+        la.set_potrf_lookahead(handle, -1)
+        la.set_potrf_leaf(handle, 0)
+        la.set_potrf_blocking(handle, 0, 0)
+    assert np.array_equal(outs[0], outs[1])
+    tol = (4 * n * np.finfo(np.float32).eps) if fp32 else (4 * n * np.finfo(float).eps)
+    assert synth.factor_error(A, outs[0]) < tol
+
+
 @pytest.mark.parametrize("k", [0, 5, 64, 100, 700])
 def test_potrf_info_not_positive_definite(handle, k):
     """LAPACK info: k+1 when the leading minor of order k+1 is the first that
@@ -185,7 +215,7 @@ def test_potrf_lookahead_is_bitwise_identical(handle, upper):
             outs.append(host(dA))
             assert int(info.item()) == 0
         finally:
-            la.set_potrf_lookahead(handle, True)
+            la.set_potrf_lookahead(handle, -1)
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
 
 
@@ -206,7 +236,7 @@ def test_potrf_narrow_panels(handle, fp32):
             outs.append(np.tril(host(dA)).astype(np.float64))
             assert int(info.item()) == 0
         finally:
-            la.set_potrf_lookahead(handle, True)
+            la.set_potrf_lookahead(handle, -1)
     assert np.array_equal(outs[0], outs[1])
     tol = (4 * n * np.finfo(np.float32).eps) if fp32 else (4 * n * np.finfo(float).eps)
     e_narrow, e_wide = synth.factor_error(A, outs[0]), synth.factor_error(A, outs[2])
