@@ -18,13 +18,13 @@
 //      LDS (P) as the operand of
 //   3. the trailing update A22 -= P P^T on the lower triangle, 16 x 16 MFMA
 //      tiles spread over the four waves, each wave prefetching its next tile.
-// The forward substitution with L runs inside the factorisation, on each
-// block column while its factor is in LDS (diagonal block solved per
+// In FP64 the forward substitution with L runs inside the factorisation, on
+// each block column while its factor is in LDS (diagonal block solved per
 // right-hand side, rows below updated one row per thread); the backward
-// substitution with L^T follows (rows above updated one row per thread).  A
-// solve with a given factor runs both sweeps the same way.  The
-// factorisation and the small solves are FP64; the trailing update is on
-// MFMA in T.
+// substitution with L^T follows (rows above updated one row per thread).  In
+// FP32 both sweeps follow the factorisation, on 32-row blocks.  A solve with
+// a given factor runs the sweeps the same way.  The factorisation and the
+// small solves are FP64; the trailing update is on MFMA in T.
 #include <algorithm>
 #include <type_traits>
 #include <vector>
@@ -103,6 +103,15 @@ __shared__ Smem<T, NB> g_bsm;
 
 template <int NB, int LD, int XL>
 __device__ __forceinline__ void diag_solve(const double* L, double* X, int nrhs, bool forward, int tid);
+
+// Block rows of the substitution sweeps of the solve (see solve_one_impl),
+// and whether the forward sweep runs inside the factorisation instead, on
+// each block column while its rows are in LDS (FP64: 12.5 -> 11.0 ms when
+// introduced; FP32 is faster with both sweeps on 32-row blocks after it).
+template <typename T, int NB>
+constexpr int batched_solve_block() { return sizeof(T) == 8 ? NB : 32; }
+template <typename T>
+constexpr bool batched_fused_forward() { return sizeof(T) == 8; }
 
 // Factor the n x n matrix A (column-major, ld) in place.  Returns 0 or the
 // LAPACK info (k+1 of the first failing pivot).  Uniform across the block.
@@ -341,55 +350,64 @@ __device__ __forceinline__ void solve_one_impl(const T* A, int n, int64_t lda, T
         Y[i * YL + j] = B[i + (int64_t)j * ldb];
     }
     __syncthreads();
-    // diagonal block of BS rows at k (identity-padded) into Ld (ld BS + 1),
-    // its right-hand sides into Xd
-    auto load_diag = [&](auto bs_c, int k, int kb, double* Ld, double* Xd) {
-        constexpr int BS = decltype(bs_c)::value, LDB = BS + 1;
-        for (int e = tid; e < BS * BS; e += kThreads) {
-            const int i = e % BS, c = e / BS;
-            Ld[i * LDB + c] = (i < kb && c <= i) ? (double)A[(k + i) + (int64_t)(k + c) * lda] : (i == c ? 1.0 : 0.0);
+    // Both sweeps run on SB-row blocks.  A block step costs two memory round
+    // trips and three barriers whatever its width, so FP32 takes 32 rows
+    // (half the steps of NB = 16: 7.55 -> 6.83 ms on the 4096-system
+    // benchmark, with the forward sweep here rather than inside the
+    // factorisation); FP64 keeps the factorisation's NB (32 rows measured
+    // 9.1-9.4 against 8.5 ms there: the out-of-line routine pays for the
+    // wider row registers).  The block and its right-hand sides live in the
+    // factorisation's L and X when SB = NB, in its deferred-update buffer P2
+    // (free once the factor is done) otherwise.
+    constexpr int SB = batched_solve_block<T, NB>(), SLD = SB + 1;
+    static_assert(SB == NB || sizeof(s.P2) >= sizeof(double) * SB * (SLD + XL), "solve buffers");
+    double* const SL = SB == NB ? s.L : reinterpret_cast<double*>(s.P2);
+    double* const SX = SB == NB ? s.X : reinterpret_cast<double*>(s.P2) + SB * SLD;
+    // diagonal block of SB rows at k (identity-padded) into SL, its
+    // right-hand sides into SX
+    auto load_diag = [&](int k, int kb) {
+        for (int e = tid; e < SB * SB; e += kThreads) {
+            const int i = e % SB, c = e / SB;
+            SL[i * SLD + c] = (i < kb && c <= i) ? (double)A[(k + i) + (int64_t)(k + c) * lda] : (i == c ? 1.0 : 0.0);
         }
-        for (int e = tid; e < BS * nrhs; e += kThreads) {
-            const int i = e % BS, j = e / BS;
-            Xd[i * XL + j] = (i < kb) ? (double)Y[(k + i) * YL + j] : 0.0;
+        for (int e = tid; e < SB * nrhs; e += kThreads) {
+            const int i = e % SB, j = e / SB;
+            SX[i * XL + j] = (i < kb) ? (double)Y[(k + i) * YL + j] : 0.0;
         }
     };
-    // forward: L y = b, in the factorisation's NB blocks (the fused
-    // forward substitution of factor_one_impl gives the same Y bit for bit)
-    for (int k = 0; k < n && !forward_done; k += NB) {
-        constexpr int LD = NB + 1;
-        const int kb = min(NB, n - k), m = n - k - kb;
-        load_diag(std::integral_constant<int, NB>{}, k, kb, s.L, s.X);
+    const int nblk = (n + SB - 1) / SB;
+    // forward: L y = b
+    for (int q = 0; q < nblk && !forward_done; ++q) {
+        const int k = q * SB, kb = min(SB, n - k), m = n - k - kb;
+        load_diag(k, kb);
         __syncthreads();
-        diag_solve<NB, LD, XL>(s.L, s.X, nrhs, true, tid);  // L_kk y = b
+        diag_solve<SB, SLD, XL>(SL, SX, nrhs, true, tid);  // L_kk y = b
         __syncthreads();
         for (int e = tid; e < kb * nrhs; e += kThreads) {
             const int i = e % kb, j = e / kb;
-            Y[(k + i) * YL + j] = (T)s.X[i * XL + j];
+            Y[(k + i) * YL + j] = (T)SX[i * XL + j];
         }
         for (int r = tid; r < m; r += kThreads) {  // b_r -= L[r, blk] y_blk
             const T* lrow = A + (k + kb + r) + (int64_t)k * lda;
-            double l[NB];
+            double l[SB];
 #pragma unroll
-            for (int c = 0; c < NB; ++c) l[c] = (c < kb) ? (double)lrow[(int64_t)c * lda] : 0.0;
+            for (int c = 0; c < SB; ++c) l[c] = (double)lrow[(int64_t)min(c, kb - 1) * lda];
+#pragma unroll
+            for (int c = 0; c < SB; ++c) l[c] = c < kb ? l[c] : 0.0;
             for (int j = 0; j < nrhs; ++j) {
                 double acc = 0.0;
 #pragma unroll
-                for (int c = 0; c < NB; ++c) acc = fma(l[c], s.X[c * XL + j], acc);
-                T* b = &Y[(k + kb + r) * YL + j];
-                *b = (T)((double)*b - acc);
+                for (int c = 0; c < SB; ++c) acc = fma(l[c], SX[c * XL + j], acc);
+                T* y = &Y[(k + kb + r) * YL + j];
+                *y = (T)((double)*y - acc);
             }
         }
         __syncthreads();
     }
     // backward: L^T x = y, last block first
-    constexpr int SB = NB, SLD = SB + 1;
-    double* const SL = s.L;
-    double* const SX = s.X;
-    const int nblk = (n + SB - 1) / SB;
     for (int q = nblk - 1; q >= 0; --q) {
         const int k = q * SB, kb = min(SB, n - k);
-        load_diag(std::integral_constant<int, SB>{}, k, kb, SL, SX);
+        load_diag(k, kb);
         __syncthreads();
         diag_solve<SB, SLD, XL>(SL, SX, nrhs, false, tid);  // L_kk^T x = y
         __syncthreads();
@@ -401,13 +419,15 @@ __device__ __forceinline__ void solve_one_impl(const T* A, int n, int64_t lda, T
             const T* lcol = A + k + (int64_t)r * lda;  // L(k + c, r), contiguous in c
             double l[SB];
 #pragma unroll
-            for (int c = 0; c < SB; ++c) l[c] = (c < kb) ? (double)lcol[c] : 0.0;
+            for (int c = 0; c < SB; ++c) l[c] = (double)lcol[min(c, kb - 1)];
+#pragma unroll
+            for (int c = 0; c < SB; ++c) l[c] = c < kb ? l[c] : 0.0;
             for (int j = 0; j < nrhs; ++j) {
                 double acc = 0.0;
 #pragma unroll
                 for (int c = 0; c < SB; ++c) acc = fma(l[c], SX[c * XL + j], acc);
-                T* b = &Y[r * YL + j];
-                *b = (T)((double)*b - acc);
+                T* y = &Y[r * YL + j];
+                *y = (T)((double)*y - acc);
             }
         }
         __syncthreads();
@@ -454,13 +474,15 @@ __device__ __forceinline__ void batched_posv_body(const BArgs<T>& a) {
             const int n = (int)n64;
             if (a.factor) {
                 // the forward substitution runs inside the factorisation
-                T* const b = a.nrhs > 0 ? a.B[mat] : nullptr;
+                constexpr bool fuse = batched_fused_forward<T>();
+                T* const b = a.nrhs > 0 && fuse ? a.B[mat] : nullptr;
+                const int nr = fuse ? a.nrhs : 0;
                 const int64_t ldb = a.nrhs > 0 ? a.ldb[mat] : 1;
-                if constexpr (INL) info = factor_one_impl<T, NB>(a.A[mat], n, a.lda[mat], b, a.nrhs, ldb);
-                else info = factor_one<T, NB>(a.A[mat], n, a.lda[mat], b, a.nrhs, ldb);
+                if constexpr (INL) info = factor_one_impl<T, NB>(a.A[mat], n, a.lda[mat], b, nr, ldb);
+                else info = factor_one<T, NB>(a.A[mat], n, a.lda[mat], b, nr, ldb);
             }
             if (info == 0 && a.nrhs > 0) {
-                const bool fwd = a.factor != 0;
+                const bool fwd = a.factor != 0 && batched_fused_forward<T>();
                 if constexpr (INL) solve_one_impl<T, NB>(a.A[mat], n, a.lda[mat], a.B[mat], a.nrhs, a.ldb[mat], fwd);
                 else solve_one<T, NB>(a.A[mat], n, a.lda[mat], a.B[mat], a.nrhs, a.ldb[mat], fwd);
             }
