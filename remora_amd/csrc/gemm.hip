@@ -132,6 +132,44 @@ __device__ __forceinline__ void lower_trapezoid_tile(int64_t t, int64_t tn, int6
     }
 }
 
+// The same tiles in groups of kGroupT tile rows, each group column by
+// column: tile rows [i0, i0 + g) take the tiles (i0 .. i0+g-1, j) for
+// j = 0 .. i0-1, then their g x g diagonal block column by column.  An XCD
+// works through a contiguous range of this order, so its 64 concurrent
+// tiles form a block of about 8 x 8 tiles and share 8 row and 8 column
+// blocks of the operands (row order shares one row block and reads 64
+// column blocks).  The rectangular part below a triangle stays row by row
+// (rows of tn tiles).
+constexpr int kGroupT = 8;
+__device__ __forceinline__ void lower_trapezoid_tile_grouped(int64_t t, int64_t tn, int64_t& ti, int64_t& tj) {
+    const int64_t tri = tn * (tn + 1) / 2;
+    if (t >= tri) {
+        const int64_t r = t - tri;
+        ti = tn + r / tn;
+        tj = r % tn;
+        return;
+    }
+    int64_t i = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);  // row of t in row order
+    while (i * (i + 1) / 2 > t) --i;
+    while ((i + 1) * (i + 2) / 2 <= t) ++i;
+    const int64_t i0 = (i / kGroupT) * kGroupT;  // first row of its group
+    const int64_t g = min((int64_t)kGroupT, tn - i0);
+    int64_t r = t - i0 * (i0 + 1) / 2;
+    if (r < i0 * g) {
+        tj = r / g;
+        ti = i0 + r % g;
+        return;
+    }
+    r -= i0 * g;
+    int64_t jj = 0;
+    while (r >= g - jj) {
+        r -= g - jj;
+        ++jj;
+    }
+    tj = i0 + jj;
+    ti = i0 + jj + r;
+}
+
 // SYM != 0: op(A) is a symmetric matrix of which only the lower (SYM == 1)
 // or upper (SYM == 2) triangle is stored (requires !TA).  Tiles wholly in
 // the stored triangle load directly, tiles wholly in the other one load the
@@ -193,9 +231,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         tm = first + r % gsz;
         tn = r / gsz;
     } else if (PART == 1) {
-        lower_trapezoid_tile(pid, p.tiles_n, tm, tn);
+        if (p.k_from_row) lower_trapezoid_tile(pid, p.tiles_n, tm, tn);
+        else lower_trapezoid_tile_grouped(pid, p.tiles_n, tm, tn);
     } else {
-        lower_trapezoid_tile(pid, p.tiles_m, tn, tm);
+        if (p.k_from_row) lower_trapezoid_tile(pid, p.tiles_m, tn, tm);
+        else lower_trapezoid_tile_grouped(pid, p.tiles_m, tn, tm);
     }
     // the tile origin is uniform: keep it in scalar registers so that the
     // operand bases of the K loop are scalar too
@@ -502,7 +542,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         if (pid < full) pid = (pid % kXcd) * (G / kXcd) + pid / kXcd;
     }
     int64_t tm, tn;
-    lower_trapezoid_tile(pid, p.tiles_n, tm, tn);
+    lower_trapezoid_tile_grouped(pid, p.tiles_n, tm, tn);
     const int64_t m0 = __builtin_amdgcn_readfirstlane((int)(tm * BM));
     const int64_t n0 = __builtin_amdgcn_readfirstlane((int)(tn * BN));
     const int64_t M = p.M, N = p.N, K = p.K;
