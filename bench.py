@@ -474,6 +474,8 @@ def main():
     ap.add_argument("--nbi", type=int, default=0, help="potrf diagonal block (0: library default)")
     ap.add_argument("--no-lookahead", action="store_true", help="single-stream factorisation")
     ap.add_argument("--lookahead-mode", type=int, default=-1, help="look-ahead mode 0/1/2 (-1: library default)")
+    ap.add_argument("--reserve", type=int, default=0,
+                    help="compute units per XCD reserved for the panel launches on every look-ahead step (0: none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -492,6 +494,8 @@ def main():
         la.set_potrf_lookahead(h, False)
     if args.lookahead_mode >= 0:
         la.set_potrf_lookahead(h, args.lookahead_mode)
+    if args.reserve > 0:
+        la.set_potrf_reserve(h, args.reserve)
     paths, acc = {}, {}
 
     roof64 = roof32 = None

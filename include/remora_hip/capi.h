@@ -328,6 +328,20 @@ rmh_status rmh_set_spotrf_precision(rmh_handle h, rmh_factor_precision prec);
  * -1 (the default) picks by precision: mode 1 for rmh_dpotrf, mode 4 for
  * rmh_spotrf. */
 rmh_status rmh_set_potrf_lookahead(rmh_handle h, int enable);
+/* Look-ahead rmh_?potrf: number of compute units of each XCD (0 to 4) that
+ * the trailing updates leave free, so that the second stream's panel
+ * launches start without waiting for trailing-update workgroups to retire,
+ * for the panel steps whose trailing rows (n minus the next panel's first
+ * column) lie in (rows_lo, rows_hi].  Those steps run their updates on a
+ * handle-owned stream created with that CU mask
+ * (hipExtStreamCreateWithCUMask); the call still completes in the caller's
+ * stream order, and the factors are identical for every setting.  0 (the
+ * default) reserves nothing.  Measured at n = 16384: 4 units over
+ * (4096, 12288] shorten rmh_dpotrf alone by about 2% and (4096, 16384] the
+ * FP32 factorisation of rmh_dsposv by about 1%, while the masked updates run
+ * on 224 of the 256 units.  No counterpart in the reference (a tuning knob of
+ * this library). */
+rmh_status rmh_set_potrf_reserve(rmh_handle h, int cus_per_xcd, size_t rows_lo, size_t rows_hi);
 
 /* ---------------------------------------------------------- profiling -- */
 /* Kernel classes of rmh_?potrf: bit 0 "trailing_update" (trailing updates on

@@ -90,6 +90,7 @@ SIGNATURES = {
     "rmh_batched_autotune": (ST, [P, I, S, P, S, P, C.POINTER(D)]),
     "rmh_set_potrf_blocking": (ST, [P, S, S]),
     "rmh_set_potrf_leaf": (ST, [P, S]),
+    "rmh_set_potrf_reserve": (ST, [P, I, S, S]),
     "rmh_set_potrf_lookahead": (ST, [P, I]),
     "rmh_set_spotrf_precision": (ST, [P, I]),
     "rmh_profile_enable": (ST, [P, I]),

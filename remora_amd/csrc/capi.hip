@@ -84,6 +84,45 @@ rmh_status Context::side_stream(hipStream_t* st, hipEvent_t* ready, hipEvent_t* 
     return RMH_STATUS_SUCCESS;
 }
 
+// The stream of the look-ahead trailing updates: its CU mask leaves out
+// `rsv` compute units of each of the 8 XCDs.  Mask bit b stands for CU b of
+// the device; the bits left out are (per + 1) x + 8 y for XCD x and y < rsv
+// (per = CUs per XCD), which are rsv units of XCD x both where the bits run
+// through the XCDs in turn (bit b on XCD b % 8) and where each XCD owns a
+// contiguous range of per bits.  Returns the caller's stream unchanged when
+// the device does not divide that way.
+rmh_status Context::trail_stream(int rsv, hipStream_t* st, hipEvent_t* in, hipEvent_t* out) {
+    if (trail != nullptr && trail_rsv != rsv) {
+        RMH_TRY_HIP(hipStreamSynchronize(trail));
+        RMH_TRY_HIP(hipStreamDestroy(trail));
+        trail = nullptr;
+    }
+    if (trail == nullptr) {
+        constexpr int kXcd = 8;  // XCDs of an MI355X
+        hipDeviceProp_t prop;
+        RMH_TRY_HIP(hipGetDeviceProperties(&prop, device));
+        const int ncu = prop.multiProcessorCount, per = ncu / kXcd;
+        if (rsv <= 0 || ncu % kXcd != 0 || per % 8 != 0 || per < 8 * rsv) return RMH_STATUS_SUCCESS;
+        std::vector<uint32_t> mask((size_t)(ncu + 31) / 32, 0u);
+        for (int b = 0; b < ncu; ++b) mask[(size_t)b / 32] |= 1u << (b % 32);
+        for (int x = 0; x < kXcd; ++x)
+            for (int y = 0; y < rsv; ++y) {
+                const int b = (per + 1) * x + 8 * y;
+                mask[(size_t)b / 32] &= ~(1u << (b % 32));
+            }
+        RMH_TRY_HIP(hipExtStreamCreateWithCUMask(&trail, (uint32_t)mask.size(), mask.data()));  // default priority
+        trail_rsv = rsv;
+        if (ev_c == nullptr) {
+            RMH_TRY_HIP(hipEventCreateWithFlags(&ev_c, hipEventDisableTiming));
+            RMH_TRY_HIP(hipEventCreateWithFlags(&ev_d, hipEventDisableTiming));
+        }
+    }
+    *st = trail;
+    *in = ev_c;
+    *out = ev_d;
+    return RMH_STATUS_SUCCESS;
+}
+
 rmh_status Context::prof_begin(hipStream_t s, int cls, double flops, int* slot) {
     *slot = -1;
     if (((prof_mask >> cls) & 1) == 0) return RMH_STATUS_SUCCESS;
@@ -120,6 +159,14 @@ Context::~Context() {
         (void)hipStreamDestroy(side);
         (void)hipEventDestroy(ev_a);
         (void)hipEventDestroy(ev_b);
+    }
+    if (trail != nullptr) {
+        (void)hipStreamSynchronize(trail);
+        (void)hipStreamDestroy(trail);
+    }
+    if (ev_c != nullptr) {
+        (void)hipEventDestroy(ev_c);
+        (void)hipEventDestroy(ev_d);
     }
     if (ws != nullptr) (void)hipFree(ws);
     for (int i = 0; i < WS_COUNT; ++i)
@@ -477,28 +524,48 @@ rmh_status potrf_core(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
     // the columns right of panel k+1, the side stream factors panel k+1
     // (whose columns s updated first).  Both read panel k only; the column
     // ranges they write are disjoint.
-    hipStream_t s2 = nullptr;
-    hipEvent_t ev_ready = nullptr, ev_done = nullptr;
+    // With compute units reserved (rmh_set_potrf_reserve) the steps whose
+    // trailing rows fall in the reservation window run their updates on m, a
+    // stream whose CU mask leaves those units to the side stream's panel
+    // launches; `cur` is the stream of the current step's updates, and work
+    // moves between s and m behind an event.
+    hipStream_t s2 = nullptr, m = s;
+    hipEvent_t ev_ready = nullptr, ev_done = nullptr, ev_in = nullptr, ev_out = nullptr;
     if (lookahead) RMH_TRY(h->side_stream(&s2, &ev_ready, &ev_done));
+    const int rsv = h->reserve_cus;
+    const int64_t rsv_lo = (int64_t)h->rsv_rows_lo, rsv_hi = (int64_t)h->rsv_rows_hi;
+    if (lookahead && rsv > 0) RMH_TRY(h->trail_stream(rsv, &m, &ev_in, &ev_out));
+    hipStream_t cur = s;
+    auto move_to = [&](hipStream_t want) -> rmh_status {
+        if (want == cur) return RMH_STATUS_SUCCESS;
+        hipEvent_t e = want == m ? ev_in : ev_out;
+        RMH_TRY_HIP(hipEventRecord(e, cur));
+        RMH_TRY_HIP(hipStreamWaitEvent(want, e, 0));
+        cur = want;
+        return RMH_STATUS_SUCCESS;
+    };
     RMH_TRY(factor_panel(0, std::min(nb, n), s));
     for (int64_t k0 = 0; k0 < n; k0 += nb) {
         const int64_t kw = std::min(nb, n - k0);
         const int64_t k1 = k0 + kw;  // next panel
         if (k1 >= n) break;
         const int64_t kw1 = std::min(nb, n - k1);
-        RMH_TRY(trailing(k0, kw, k1, k1 + kw1, s));  // next panel's columns first
+        const int64_t rows = n - k1;
+        RMH_TRY(move_to(m != s && rows > rsv_lo && rows <= rsv_hi ? m : s));
+        RMH_TRY(trailing(k0, kw, k1, k1 + kw1, cur));  // next panel's columns first
         if (lookahead) {
-            RMH_TRY_HIP(hipEventRecord(ev_ready, s));
+            RMH_TRY_HIP(hipEventRecord(ev_ready, cur));
             RMH_TRY_HIP(hipStreamWaitEvent(s2, ev_ready, 0));
             RMH_TRY(factor_panel(k1, kw1, s2));
             RMH_TRY_HIP(hipEventRecord(ev_done, s2));
-            RMH_TRY(trailing(k0, kw, k1 + kw1, n, s));
-            RMH_TRY_HIP(hipStreamWaitEvent(s, ev_done, 0));
+            RMH_TRY(trailing(k0, kw, k1 + kw1, n, cur));
+            RMH_TRY_HIP(hipStreamWaitEvent(cur, ev_done, 0));
         } else {
-            RMH_TRY(trailing(k0, kw, k1 + kw1, n, s));
-            RMH_TRY(factor_panel(k1, kw1, s));
+            RMH_TRY(trailing(k0, kw, k1 + kw1, n, cur));
+            RMH_TRY(factor_panel(k1, kw1, cur));
         }
     }
+    RMH_TRY(move_to(s));
     return launch_scatter_diag<T>(A, rs, cs, n, (int)nbi, dfac, n, info, s);
 }
 
@@ -1014,6 +1081,16 @@ rmh_status rmh_set_potrf_blocking(rmh_handle h, size_t nb, size_t nb_inner) {
                 "set_potrf_blocking: nb (and the default widths) must be multiples of nb_inner");
     ctx(h)->nb = nb;
     ctx(h)->nbi = nb_inner;
+    return RMH_STATUS_SUCCESS;
+}
+
+rmh_status rmh_set_potrf_reserve(rmh_handle h, int cus_per_xcd, size_t rows_lo, size_t rows_hi) {
+    if (h == nullptr) return fail(RMH_STATUS_INVALID_HANDLE, "NULL handle");
+    RMH_REQUIRE(cus_per_xcd >= 0 && cus_per_xcd <= 4, "set_potrf_reserve: units per XCD must be 0 to 4");
+    RMH_REQUIRE(rows_lo <= rows_hi, "set_potrf_reserve: rows_lo above rows_hi");
+    ctx(h)->reserve_cus = cus_per_xcd;
+    ctx(h)->rsv_rows_lo = rows_lo;
+    ctx(h)->rsv_rows_hi = rows_hi;
     return RMH_STATUS_SUCCESS;
 }
 

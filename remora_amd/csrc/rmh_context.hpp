@@ -30,7 +30,6 @@ struct Context {
     static constexpr size_t kDefaultNbF64 = 1024, kDefaultNbF32 = 1024, kDefaultNbBf16x3 = 768;
     static constexpr size_t kDefaultNbi = 32;
     static constexpr size_t kDefaultNbLeaf = 128;
-
     int device = 0;
     void* ws = nullptr;  // WS_MAIN (potrf / trsm scratch)
     size_t ws_bytes = 0;
@@ -49,6 +48,15 @@ struct Context {
     bool spotrf_bf16x3 = false;  // rmh_set_spotrf_precision
     hipStream_t side = nullptr;  // look-ahead stream of potrf (default priority)
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    // look-ahead potrf: compute units per XCD kept free of the trailing
+    // updates (rmh_set_potrf_reserve), so that the panel launches of the side
+    // stream find room at once; the updates then run on `trail`, a stream
+    // whose CU mask leaves those units out (made for `trail_rsv` units)
+    int reserve_cus = 0;                       // 0: none (the default)
+    size_t rsv_rows_lo = 0, rsv_rows_hi = 0;  // the steps whose rows n - k1 lie in (lo, hi] reserve
+    hipStream_t trail = nullptr;
+    int trail_rsv = 0;
+    hipEvent_t ev_c = nullptr, ev_d = nullptr;
     std::vector<ProfRec> recs;
     // batched solver: tuned launch configuration ([0] FP32, [1] FP64) and the
     // work counters of its persistent kernel (zeroed once, kept zero by it)
@@ -59,6 +67,7 @@ struct Context {
     rmh_status reserve(size_t bytes);
     rmh_status reserve_slot(int slot, size_t bytes, void** ptr);
     rmh_status side_stream(hipStream_t* st, hipEvent_t* ready, hipEvent_t* done);
+    rmh_status trail_stream(int rsv, hipStream_t* st, hipEvent_t* in, hipEvent_t* out);
     rmh_status prof_begin(hipStream_t s, int cls, double flops, int* slot);
     rmh_status prof_end(hipStream_t s, int slot);
     void prof_clear();

@@ -268,6 +268,13 @@ def set_potrf_leaf(h, leaf=0):
     capi.check(capi.load().rmh_set_potrf_leaf(h.value, leaf))
 
 
+def set_potrf_reserve(h, cus_per_xcd=0, rows_lo=0, rows_hi=1 << 62):
+    """Compute units per XCD the look-ahead trailing updates leave to the
+    panel launches (0: none, the default), for the panel steps whose
+    trailing rows n - k1 lie in (rows_lo, rows_hi]."""
+    capi.check(capi.load().rmh_set_potrf_reserve(h.value, cus_per_xcd, rows_lo, rows_hi))
+
+
 def set_spotrf_precision(h, prec="fp32"):
     p = {"fp32": capi.FACTOR_FP32, "bf16x3": capi.FACTOR_BF16X3}[prec]
     capi.check(capi.load().rmh_set_spotrf_precision(h.value, p))

@@ -220,6 +220,41 @@ def test_potrf_lookahead_is_bitwise_identical(handle, upper):
 
 
 @pytest.mark.parametrize("fp32", [False, True])
+@pytest.mark.parametrize("upper", [False, True])
+def test_potrf_cu_reserve_is_bitwise_identical(handle, upper, fp32):
+    """Reserving compute units moves panel steps' trailing updates to a
+    CU-masked stream: every step, a window of them, or none.  The launches
+    and their order on each column range are unchanged, so the factor is
+    bit-identical in every case.  With n = 5200 and panels of 1024 the steps
+    have n - k1 = 4176, 3152, 2128, 1104 trailing rows, so the window
+    (2500, 3500] moves one step to the masked stream and back."""
+    from remora_amd import linalg as la
+    n = 5200
+    A = synth.gram_spd(n, seed=13)
+    outs = []
+    try:
+        for cus, lo, hi in ((0, 0, 0), (4, 0, 1 << 62), (2, 2500, 3500), (4, 4096, 12288)):
+            la.set_potrf_reserve(handle, cus, lo, hi)
+            dA = dev(A.astype(np.float32) if fp32 else A)
+            info = la.potrf(handle, dA, upper=upper)
+            assert int(info.item()) == 0
+            outs.append(host(dA))
+    finally:
+        la.set_potrf_reserve(handle, 0)
+    for o in outs[1:]:
+        assert np.array_equal(outs[0], o)
+    tol = (4 * n * np.finfo(np.float32).eps) if fp32 else (4 * n * np.finfo(float).eps)
+    L = np.triu(outs[0]).T if upper else np.tril(outs[0])
+    assert synth.factor_error(A, L.astype(np.float64)) < tol
+    with pytest.raises(Exception):
+        la.set_potrf_reserve(handle, 5)
+    with pytest.raises(Exception):
+        la.set_potrf_reserve(handle, -1)
+    with pytest.raises(Exception):
+        la.set_potrf_reserve(handle, 2, 100, 10)
+
+
+@pytest.mark.parametrize("fp32", [False, True])
 def test_potrf_narrow_panels(handle, fp32):
     """Modes 3 and 4 run the panel on four wavefronts (one K group): the two
     schedules give the same factor bit for bit, and it is as accurate as the
