@@ -508,6 +508,7 @@ rmh_status potrf_core(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
         const double flops = 2.0 * kw * ((double)cols * (cols + 1) / 2.0 + (double)(rows - cols) * cols);
         const Tile tile = pick_tile(g.M, g.N, g.part);
         g.one_per_cu = lookahead && share_cus && tile == Tile::T128x128;
+        g.tail_split = true;
         int slot;
         RMH_TRY(prof_begin(st, tile == Tile::T128x128 ? PROF_TRAILING : PROF_TRAILING_SMALL, flops, &slot));
         SplitPanel sp;
@@ -674,6 +675,7 @@ rmh_status syrk_entry(rmh_handle h, int upper, int trans, size_t n, size_t k, T 
     g.A = A; g.lda = (int64_t)lda; g.transA = trans != 0;
     g.B = A; g.ldb = (int64_t)lda; g.transB = trans == 0;
     g.C = C; g.ldc = (int64_t)ldc;
+    g.tail_split = true;
     return launch_gemm<T>(g, pick_tile(g.M, g.N, g.part), hs(stream));
 }
 

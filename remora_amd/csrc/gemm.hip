@@ -31,6 +31,8 @@
 // tile column (BN >= N) (or a single tile row), every workgroup reads all of
 // its own rows (columns) before it writes them and no other workgroup reads
 // them.  The Cholesky panel solve uses that (A21 <- A21 * W^T).
+#include <atomic>
+
 #include "rmh_internal.hpp"
 
 namespace rmh {
@@ -85,6 +87,10 @@ struct KParams {
     // every element offset inside a staged tile fits 32 bits (in bytes):
     // full stages load from a wave-uniform base plus fixed per-lane offsets
     int fast32;
+    // quarter tiles (PART 1 / 2, sub_tbig > 0): workgroup w computes quarter
+    // w % 4 of tile sub_t0 + w / 4 of the grouped trapezoid order on tiles of
+    // twice this kernel's size, sub_tbig of them along the short side
+    int64_t sub_t0, sub_tbig;
 };
 
 // word 3 of a raw (untyped) buffer descriptor on gfx9-family parts
@@ -230,6 +236,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         const int64_t r = pid % width;
         tm = first + r % gsz;
         tn = r / gsz;
+    } else if (p.sub_tbig > 0) {
+        int64_t bi, bj;  // the big tile in (long side, short side) order
+        lower_trapezoid_tile_grouped(p.sub_t0 + pid / 4, p.sub_tbig, bi, bj);
+        const int64_t qi = 2 * bi + ((pid >> 1) & 1), qj = 2 * bj + (pid & 1);
+        tm = PART == 1 ? qi : qj;
+        tn = PART == 1 ? qj : qi;
     } else if (PART == 1) {
         if (p.k_from_row) lower_trapezoid_tile(pid, p.tiles_n, tm, tn);
         else lower_trapezoid_tile_grouped(pid, p.tiles_n, tm, tn);
@@ -241,6 +253,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     // operand bases of the K loop are scalar too
     const int64_t m0 = __builtin_amdgcn_readfirstlane((int)(tm * BM));
     const int64_t n0 = __builtin_amdgcn_readfirstlane((int)(tn * BN));
+    if (PART != 0 && p.sub_tbig > 0) {
+        // quarters past the matrix edge or wholly outside the trapezoid
+        // (the whole workgroup leaves together)
+        if (m0 >= p.M || n0 >= p.N) return;
+        if (PART == 1 ? m0 + BM <= n0 : n0 + BN <= m0) return;
+    }
     if (p.k_from_row) {
         const int64_t kb = min(p.K, m0);  // m0 is a multiple of BK
         p.A += TA ? kb : kb * p.lda;
@@ -753,9 +771,40 @@ __global__ void split_reduce_kernel(const T* __restrict__ ws, int splits, int64_
     }
 }
 
+// the gemm_kernel instance for d's transposes on prepared parameters
+template <typename T, int BM, int BN, int BK, int PART>
+rmh_status launch_kparams(const GemmDesc<T>& d, const KParams<T>& p, dim3 grid, size_t dyn, hipStream_t stream) {
+    const dim3 block(kThreads);
+    if (!d.transA && !d.transB)
+        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, false, false, PART>), grid, block, dyn, stream, p);
+    else if (!d.transA && d.transB)
+        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, false, true, PART>), grid, block, dyn, stream, p);
+    else if (d.transA && !d.transB)
+        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, true, false, PART>), grid, block, dyn, stream, p);
+    else
+        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, true, true, PART>), grid, block, dyn, stream, p);
+    return check_launch("gemm_kernel");
+}
+
+// compute units of the current device (cached per device)
+int device_cu_count() {
+    static std::atomic<int> cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+    int c = cache[dev].load(std::memory_order_relaxed);
+    if (c == 0) {
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+            (void)hipGetLastError();
+            return 0;
+        }
+        cache[dev].store(c, std::memory_order_relaxed);
+    }
+    return c;
+}
+
 template <typename T, int BM, int BN, int BK, int PART>
 rmh_status launch_cfg(const GemmDesc<T>& d, hipStream_t stream) {
-    KParams<T> p;
+    KParams<T> p{};
     p.M = d.M; p.N = d.N; p.K = d.K;
     p.alpha = d.alpha; p.beta = d.beta;
     p.A = d.A; p.lda = d.lda; p.B = d.B; p.ldb = d.ldb; p.C = d.C; p.ldc = d.ldc;
@@ -786,21 +835,40 @@ rmh_status launch_cfg(const GemmDesc<T>& d, hipStream_t stream) {
     if (ntiles <= 0) return RMH_STATUS_SUCCESS;
     if (ntiles > 0x7fffffffLL) return fail(RMH_STATUS_INVALID_VALUE, "gemm: problem too large for one launch");
     p.ntiles = ntiles;
-    dim3 grid((unsigned)ntiles, (unsigned)std::max(1, d.splits), (unsigned)std::max<int64_t>(1, d.batch)),
-        block(kThreads);
+    // Trapezoid on 128 x 128 tiles whose last round of workgroups would fill
+    // only part of the device: the full rounds run here, and the remaining
+    // tiles of the order run as 64 x 64 quarters in a second launch, four
+    // times as many workgroups of a quarter of the work (four fit on a
+    // compute unit where two 128 x 128 ones do).  Both tile shapes form each
+    // element's sum in the same order (same MFMA, same depth step, same
+    // epilogue), so the result is bitwise the same.  The FP64 quarters keep
+    // most of the large tiles' MFMA rate and pay off up to half a round; the
+    // FP32 ones run at about half of it (38% against 74% busy), so four of
+    // them per compute unit take as long as the two tiles they replace and
+    // FP32 splits up to a quarter of a round only.
+    if constexpr (BM == 128 && BN == 128 && PART != 0) {
+        const int64_t slots = 2 * (int64_t)device_cu_count();  // two workgroups per compute unit
+        const int64_t rounds = ntiles / slots, rest = ntiles - rounds * slots;
+        const int64_t max_rest = sizeof(T) == 8 ? slots / 2 : slots / 4;
+        if (d.tail_split && d.splits <= 1 && d.batch <= 1 && !d.k_from_row && !d.one_per_cu && slots > 0 &&
+            rounds >= 1 && rest > 0 && rest <= max_rest) {
+            p.ntiles = rounds * slots;
+            RMH_TRY((launch_kparams<T, 128, 128, BK, PART>(d, p, dim3((unsigned)p.ntiles), 0, stream)));
+            KParams<T> q = p;
+            q.tiles_m = (d.M + 63) / 64;
+            q.tiles_n = (d.N + 63) / 64;
+            q.sub_t0 = rounds * slots;
+            q.sub_tbig = PART == 1 ? p.tiles_n : p.tiles_m;
+            q.ntiles = 4 * rest;
+            return launch_kparams<T, 64, 64, BK, PART>(d, q, dim3((unsigned)q.ntiles), 0, stream);
+        }
+    }
+    const dim3 grid((unsigned)ntiles, (unsigned)std::max(1, d.splits), (unsigned)std::max<int64_t>(1, d.batch));
     // one workgroup per compute unit: pad the LDS request past half of it
     constexpr size_t kStatic = 2 * BK * (BM + kPad) * sizeof(T) + 2 * BK * (BN + kPad) * sizeof(T);
     constexpr size_t kOnePerCu = 96 * 1024;
     const size_t dyn = (d.one_per_cu && kStatic < kOnePerCu) ? kOnePerCu - kStatic : 0;
-    if (!d.transA && !d.transB)
-        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, false, false, PART>), grid, block, dyn, stream, p);
-    else if (!d.transA && d.transB)
-        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, false, true, PART>), grid, block, dyn, stream, p);
-    else if (d.transA && !d.transB)
-        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, true, false, PART>), grid, block, dyn, stream, p);
-    else
-        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, true, true, PART>), grid, block, dyn, stream, p);
-    RMH_TRY(check_launch("gemm_kernel"));
+    RMH_TRY((launch_kparams<T, BM, BN, BK, PART>(d, p, grid, dyn, stream)));
     if (d.splits > 1) {
         const int64_t total = d.M * d.N;
         const unsigned g = (unsigned)std::min<int64_t>((total + 255) / 256, 4096);
@@ -823,7 +891,7 @@ rmh_status launch_part(const GemmDesc<T>& d, hipStream_t stream) {
 
 template <typename T, int BM, int BN, int BK, int SYM>
 rmh_status launch_symm_cfg(const GemmDesc<T>& d, hipStream_t stream) {
-    KParams<T> p;
+    KParams<T> p{};
     p.M = d.M; p.N = d.N; p.K = d.K;
     p.alpha = d.alpha; p.beta = d.beta;
     p.A = d.A; p.lda = d.lda; p.B = d.B; p.ldb = d.ldb; p.C = d.C; p.ldc = d.ldc;

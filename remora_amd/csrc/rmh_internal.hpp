@@ -71,6 +71,10 @@ struct GemmDesc {
     // unit's share for a 128 x 128 tile and leaves room for one workgroup of
     // a concurrent kernel (the look-ahead panel kernel)
     bool one_per_cu = false;
+    // a triangular part on 128 x 128 tiles whose last round of workgroups
+    // would fill at most half the device runs that round as 64 x 64 quarter
+    // tiles in a second launch (bitwise the same result)
+    bool tail_split = false;
 };
 
 // Launch C = alpha*op(A)*op(B) + beta*C (restricted to `part`).  `tile`

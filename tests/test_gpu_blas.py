@@ -133,6 +133,33 @@ def test_syrk_random(handle, upper, trans, nk, prec):
     assert np.array_equal(got[~mask], C0[~mask])
 
 
+@pytest.mark.parametrize("prec", ["d", "s"])
+@pytest.mark.parametrize("upper", [False, True])
+def test_syrk_tail_quarters_bitwise(handle, upper, prec):
+    """n = 9216 on 128 x 128 tiles gives 72 * 73 / 2 = 2628 trapezoid tiles:
+    five full rounds of 512 workgroups and 68 tiles that run as 64 x 64
+    quarters in a second launch.  Each element's sum is formed the same way
+    by both tile shapes, so the triangle equals the one a full 128 x 128 GEMM
+    C - E E^T writes, bit for bit (and the other triangle is untouched)."""
+    import torch
+    from remora_amd import linalg as la
+    n, k = 9216, 384
+    dt = torch.float64 if prec == "d" else torch.float32
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    E = torch.randn(k, n, generator=g, device="cuda", dtype=dt).t()  # column-major n x k
+    C0 = torch.randn(n, n, generator=g, device="cuda", dtype=dt).t()
+    C1 = C0.clone(memory_format=torch.preserve_format)
+    C2 = C0.clone(memory_format=torch.preserve_format)
+    la.syrk(handle, E, C1, alpha=-1.0, beta=1.0, upper=upper)
+    la.gemm(handle, E, E, C2, alpha=-1.0, beta=1.0, transB=True)
+    torch.cuda.synchronize()
+    tri = torch.triu if upper else torch.tril
+    other = (lambda x: torch.tril(x, -1)) if upper else (lambda x: torch.triu(x, 1))
+    assert torch.equal(tri(C1), tri(C2))
+    assert torch.equal(other(C1), other(C0))
+
+
 def test_ssyrk_reference_hip_fixture(handle):
     """Test/hip_syrk.cpp:17-98 (float, dims 936, K 1039).  The reference
     compares against a float inner product on the CPU at 1e-4 percent, i.e.
