@@ -16,13 +16,19 @@ namespace {
 
 constexpr int kT = 64;  // conversion tile
 
-// Convert the stored triangle of A (FP64) to FP32 in A32 and accumulate the
-// full-row absolute sums of the symmetric matrix into rowsum (double,
-// zeroed beforehand): a stored off-diagonal a_ij counts for rows i and j.
-// One workgroup per 64 x 64 tile of the stored triangle (lower-tile order).
+// Convert the stored triangle of A (FP64) to FP32 in A32 and form the
+// partial absolute row sums of the symmetric matrix: a stored off-diagonal
+// a_ij counts for rows i and j.  One workgroup per 64 x 64 tile of the
+// stored triangle (lower-tile order).  The tile on row block R and column
+// block C writes, for each row r of block R, the sum over its columns into
+// rsp[C * n + r], and for each column c of block C, the sum over its rows
+// (off-diagonal entries) into rsp[R * n + c] -- on a diagonal tile both go
+// into one slot.  Every slot (s, r), s < number of blocks, has exactly one
+// writer, so rowsum_kernel adds them in a fixed order: no atomics, and the
+// same ||A||_inf on every run.
 __global__ __launch_bounds__(256) void convert_sym_kernel(const double* __restrict__ A, int64_t lda,
                                                           float* __restrict__ A32, int64_t ld32, int64_t n,
-                                                          bool upper, double* __restrict__ rowsum) {
+                                                          bool upper, double* __restrict__ rsp) {
     __shared__ double mag[kT][kT + 1];  // |a| of the stored off-diagonal entries, [row][col]
     __shared__ double rpart[4][kT], cpart[4][kT];
     // tile (ti, tj), ti >= tj, in the lower-tile enumeration
@@ -32,19 +38,26 @@ __global__ __launch_bounds__(256) void convert_sym_kernel(const double* __restri
     while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
     const int64_t tj = t - ti * (ti + 1) / 2;
     // lower storage: rows ti, cols tj.  upper storage: rows tj, cols ti.
-    const int64_t r0 = (upper ? tj : ti) * kT, c0 = (upper ? ti : tj) * kT;
+    const int64_t rb = upper ? tj : ti, cb = upper ? ti : tj;
+    const int64_t r0 = rb * kT, c0 = cb * kT;
     const int tid = threadIdx.x;
     const int lr = tid & 63, lc0 = tid >> 6;  // 64 rows x 4 column phases
-    double rsum = 0.0;
     const int64_t gi = r0 + lr;
-    for (int lc = lc0; lc < kT; lc += 4) {
+    constexpr int kQ = kT / 4;
+    double v[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {  // every load in flight before any is used
+        const int64_t gj = c0 + lc0 + 4 * q;
+        const bool ok = gi < n && gj < n && (upper ? gi <= gj : gi >= gj);
+        v[q] = ok ? A[gi + gj * lda] : 0.0;
+    }
+    double rsum = 0.0;
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const int lc = lc0 + 4 * q;
         const int64_t gj = c0 + lc;
-        double a = 0.0;
-        if (gi < n && gj < n && (upper ? gi <= gj : gi >= gj)) {
-            const double v = A[gi + gj * lda];
-            A32[gi + gj * ld32] = (float)v;
-            a = fabs(v);
-        }
+        if (gi < n && gj < n && (upper ? gi <= gj : gi >= gj)) A32[gi + gj * ld32] = (float)v[q];
+        const double a = fabs(v[q]);
         rsum += a;
         mag[lr][lc] = gi != gj ? a : 0.0;  // an off-diagonal a_ij also counts for row j
     }
@@ -58,14 +71,30 @@ __global__ __launch_bounds__(256) void convert_sym_kernel(const double* __restri
         cpart[q][c] = cs;
     }
     __syncthreads();
-    if (tid < kT) {
-        const double v = (rpart[0][tid] + rpart[1][tid]) + (rpart[2][tid] + rpart[3][tid]);
-        if (r0 + tid < n && v != 0.0) atomicAdd(&rowsum[r0 + tid], v);
+    if (rb == cb) {
+        if (tid < kT) {
+            const double v2 = ((rpart[0][tid] + rpart[1][tid]) + (rpart[2][tid] + rpart[3][tid])) +
+                              ((cpart[0][tid] + cpart[1][tid]) + (cpart[2][tid] + cpart[3][tid]));
+            if (r0 + tid < n) rsp[cb * n + r0 + tid] = v2;
+        }
+    } else if (tid < kT) {
+        const double v2 = (rpart[0][tid] + rpart[1][tid]) + (rpart[2][tid] + rpart[3][tid]);
+        if (r0 + tid < n) rsp[cb * n + r0 + tid] = v2;
     } else if (tid < 2 * kT) {
         const int c = tid - kT;
-        const double v = (cpart[0][c] + cpart[1][c]) + (cpart[2][c] + cpart[3][c]);
-        if (c0 + c < n && v != 0.0) atomicAdd(&rowsum[c0 + c], v);
+        const double v2 = (cpart[0][c] + cpart[1][c]) + (cpart[2][c] + cpart[3][c]);
+        if (c0 + c < n) rsp[rb * n + c0 + c] = v2;
     }
+}
+
+// rowsum[r] = sum over s < nb of rsp[s * n + r], s in order
+__global__ __launch_bounds__(256) void rowsum_kernel(const double* __restrict__ rsp, int64_t n, int64_t nb,
+                                                     double* __restrict__ rowsum) {
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= n) return;
+    double acc = 0.0;
+    for (int64_t sl = 0; sl < nb; ++sl) acc += rsp[sl * n + r];
+    rowsum[r] = acc;
 }
 
 __global__ __launch_bounds__(256) void max_kernel(const double* __restrict__ v, int64_t n, double* __restrict__ out) {
@@ -154,6 +183,7 @@ struct MixedWs {
     int64_t* info;
     void* inv_ws;  // block inverses of the FP32 factor (blocked solves), or NULL
     double* part;  // split-K partials of the residual product
+    double* rsp;   // partial row sums of |A| (blocks x n) from the conversion
 };
 
 constexpr int kResidualSplits = 8;
@@ -165,8 +195,9 @@ static rmh_status mixed_ws(Context* h, int64_t n, int64_t nrhs, MixedWs* w) {
     const size_t rs = align256((size_t)n * sizeof(double));
     const size_t iv = trsm_blocked_applies(n, nrhs) ? align256(trsm_inverse_workspace<float>(n, nrhs)) : 0;
     const size_t pt = align256((size_t)kResidualSplits * n * nrhs * sizeof(double));
+    const size_t rp = align256((size_t)((n + kT - 1) / kT) * n * sizeof(double));
     void* base = nullptr;
-    RMH_TRY(h->reserve_slot(WS_MIXED, a + c + r + rs + 1024 + iv + pt, &base));
+    RMH_TRY(h->reserve_slot(WS_MIXED, a + c + r + rs + 1024 + iv + pt + rp, &base));
     char* p = static_cast<char*>(base);
     w->A32 = reinterpret_cast<float*>(p); p += a;
     w->C32 = reinterpret_cast<float*>(p); p += c;
@@ -179,6 +210,8 @@ static rmh_status mixed_ws(Context* h, int64_t n, int64_t nrhs, MixedWs* w) {
     w->inv_ws = iv ? static_cast<void*>(p) : nullptr;
     p += iv;
     w->part = reinterpret_cast<double*>(p);
+    p += pt;
+    w->rsp = reinterpret_cast<double*>(p);
     return RMH_STATUS_SUCCESS;
 }
 
@@ -220,11 +253,12 @@ rmh_status dsposv_impl(Context* h, bool upper, rmh_factor_precision prec, int64_
     MixedWs w;
     RMH_TRY(mixed_ws(h, n, nrhs, &w));
     // 1. FP32 copy of the stored triangle and ||A||_inf
-    RMH_TRY(launch_scale<double>(w.rowsum, n, n, 1, 0.0, s));
     const int64_t nt = (n + kT - 1) / kT;
     hipLaunchKernelGGL(convert_sym_kernel, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), 0, s, A, lda, w.A32, n, n,
-                       upper, w.rowsum);
+                       upper, w.rsp);
     RMH_TRY(check_launch("convert_sym_kernel"));
+    hipLaunchKernelGGL(rowsum_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w.rsp, n, nt, w.rowsum);
+    RMH_TRY(check_launch("rowsum_kernel"));
     hipLaunchKernelGGL(max_kernel, dim3(1), dim3(256), 0, s, w.rowsum, n, w.anorm);
     RMH_TRY(check_launch("max_kernel"));
     // 2. factor in reduced precision (FP64 diagonal blocks, FP32 MFMA updates)
