@@ -383,6 +383,45 @@ def run_pstrf(h, steps, warmup, n=4096):
                          "frac": achieved / PEAK["f64"]}}
 
 
+def run_blas3(h, steps, warmup, n=8192):
+    """The standalone FP64 products of the drop-in configuration: GEMM
+    C += A B (m = n = k = 8192, the reference's noalias(C) += prod(A, B)) and
+    SYRK C += A A^T on the lower triangle (n = k = 8192), with A, B, C
+    N(0,1) (seed 3).  Work 2 n^3 and n^2 (n + 1) flops (the usual count; the
+    reference's benchmark programs report half of it).  Accuracy: one beta = 0
+    product, 64 sampled columns against torch's FP64 product, max relative to
+    the column's max |entry|."""
+    import torch
+    from remora_amd import linalg as la
+    g = torch.Generator(device="cuda").manual_seed(3)
+    A = la.colmajor(torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g))
+    B = la.colmajor(torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g))
+    C = la.colmajor(torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g))
+    idx = torch.randperm(n, generator=torch.Generator().manual_seed(4))[:64].cuda()
+    out = {}
+
+    def rel(got, ref):
+        return float(((got - ref).abs().amax(dim=0) / ref.abs().amax(dim=0)).max().item())
+
+    la.gemm(h, A, B, C, alpha=1.0, beta=0.0)
+    err_g = rel(C[:, idx], A @ B[:, idx])
+    per, _, _ = timed(lambda: la.gemm(h, A, B, C, alpha=1.0, beta=1.0), steps, warmup, h, False)
+    ms = median(per)
+    out["gemm_f64"] = {"m": n, "n": n, "k": n, "ms": ms, "step_ms": per, "tflops": 2.0 * n ** 3 / (ms * 1e-3) / 1e12,
+                       "frac_of_peak": 2.0 * n ** 3 / (ms * 1e-3) / PEAK["f64"], "max_rel_err_sampled": err_g}
+    la.syrk(h, A, C, alpha=1.0, beta=0.0)
+    ref = A @ A[idx].t()
+    mask = torch.arange(n, device="cuda")[:, None] >= idx[None, :]  # the stored (lower) part of each column
+    got = C[:, idx]
+    err_s = float((((got - ref).abs() * mask).amax(dim=0) / (ref.abs() * mask).amax(dim=0)).max().item())
+    per, _, _ = timed(lambda: la.syrk(h, A, C, alpha=1.0, beta=1.0), steps, warmup, h, False)
+    ms = median(per)
+    work = float(n) * n * (n + 1)
+    out["syrk_f64"] = {"n": n, "k": n, "ms": ms, "step_ms": per, "tflops": work / (ms * 1e-3) / 1e12,
+                       "frac_of_peak": work / (ms * 1e-3) / PEAK["f64"], "max_rel_err_sampled": err_s}
+    return out
+
+
 def median(v):
     s = sorted(v)
     return s[len(s) // 2]
@@ -466,6 +505,8 @@ def main():
     ap.add_argument("--no-rbf", action="store_true", help="skip the RBF legs (FP64 and mixed); profiling runs only")
     ap.add_argument("--no-gram", action="store_true", help="skip the FP64 Gram-matrix leg")
     ap.add_argument("--only-fp64", action="store_true", help="FP64 legs only (no mixed leg)")
+    ap.add_argument("--no-p64", action="store_true",
+                    help="skip the drop-in legs at n = 4096 and 32768 and the standalone GEMM / SYRK")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-systems legs")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the Cholesky-adjoint leg")
     ap.add_argument("--no-update", action="store_true", help="skip the rank-one update leg")
@@ -555,6 +596,32 @@ def main():
         log(f"[bench] gram fp64: median {med:.3f} ms  {paths['gram_fp64']['tflops']:.2f} TFLOP/s  eta {eta:.2e}  "
             f"ferr {ferr:.2e}")
         del r, A, B
+        torch.cuda.empty_cache()
+
+    if not args.no_p64:
+        # the drop-in configuration at its other sizes: potrf + potrs with 64
+        # RHS on G G^T / n + 0.1 I, and the standalone GEMM and SYRK
+        for n2, st2, wu2 in ((4096, args.steps, args.warmup), (32768, 3, 1)):
+            log(f"[bench] gram inputs n={n2} nrhs={nrhs}")
+            A, B = make_gram(n2, nrhs)
+            torch.cuda.synchronize()
+            r = run_fp64(h, A, B, st2, wu2, profile=False)
+            med = median(r["step_ms"])
+            w2 = n2 ** 3 / 3.0 + 2.0 * n2 * n2 * nrhs
+            eta = backward_error(h, A, r["X"], B)
+            ferr = factor_error_sampled(A, r["L"])
+            paths[f"gram_fp64_n{n2}"] = {"tflops": w2 / (med * 1e-3) / 1e12, "ms": med, "step_ms": r["step_ms"],
+                                         "frac_of_peak": w2 / (med * 1e-3) / PEAK["f64"]}
+            acc[f"gram_fp64_n{n2}"] = {"backward_error": eta, "factor_error_sampled": ferr}
+            log(f"[bench] gram fp64 n={n2}: median {med:.3f} ms  {paths[f'gram_fp64_n{n2}']['tflops']:.2f} TFLOP/s  "
+                f"eta {eta:.2e}  ferr {ferr:.2e}")
+            del r, A, B
+            torch.cuda.empty_cache()
+        log("[bench] FP64 GEMM 8192^3 and SYRK n=k=8192")
+        for k2, v2 in run_blas3(h, args.steps, args.warmup).items():
+            paths[k2] = v2
+            log(f"[bench] {k2}: median {v2['ms']:.3f} ms  {v2['tflops']:.2f} TFLOP/s  "
+                f"({100 * v2['frac_of_peak']:.1f}% of FP64 peak)  err {v2['max_rel_err_sampled']:.2e}")
         torch.cuda.empty_cache()
 
     if not args.no_batched:
