@@ -368,6 +368,20 @@ static rmh_status launch_trailing(Context* h, const GemmDesc<T>& g, Tile tile, c
     return launch_gemm<T>(g, tile, st);
 }
 
+// Tile of the factorisation's own updates (trapezoids).  min_tiles 0: the
+// general rule (pick_tile).  Otherwise 128 x 128 tiles once the trapezoid has
+// min_tiles of them, counted exactly.  Split-BF16x3 factors switch at 256,
+// which sends the next panel's updates to the split-BF16x3 kernel instead of
+// FP32 64 x 64 tiles (13.6 -> 12.7 ms at n = 16384).  FP64 and FP32 keep the
+// general rule: an exact count from 512 on gave FP64 nothing and FP32
+// 0.2 ms, within the benchmark's noise.
+static Tile factor_tile(int64_t M, int64_t N, Part part, int64_t min_tiles) {
+    if (min_tiles <= 0) return pick_tile(M, N, part);
+    const int64_t tl = ((part == Part::Upper ? N : M) + 127) / 128, ts = ((part == Part::Upper ? M : N) + 127) / 128;
+    const int64_t t128 = ts * (ts + 1) / 2 + (tl - ts) * ts;
+    return t128 >= min_tiles ? Tile::T128x128 : Tile::T64x64;
+}
+
 template <typename T>
 size_t potrf_workspace(const Context* h, int64_t n) {
     return 256 + (size_t)std::max<int64_t>(n, 1) * h->nbi * sizeof(T);
@@ -386,6 +400,7 @@ rmh_status potrf_core(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
                                             : (sizeof(T) == 8 ? Context::kDefaultNbF64
                                                               : (bf16x3 ? Context::kDefaultNbBf16x3 : Context::kDefaultNbF32)));
     const int64_t nbi = (int64_t)h->nbi;
+    const int64_t min_tiles = bf16x3 ? 256 : 0;  // factor_tile
     // look-ahead mode: 0 single stream; 1 look-ahead; 2 with one trailing
     // workgroup per compute unit; 3 and 4: modes 2 and 1 with four-wavefront
     // panel workgroups
@@ -468,7 +483,8 @@ rmh_status potrf_core(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
         const double flops = 2.0 * sw * ((double)(c1 - c0) * (c1 - c0 + 1) / 2.0 + (double)(n - c1) * (c1 - c0));
         int slot;
         RMH_TRY(prof_begin(st, PROF_PANEL, flops, &slot));
-        RMH_TRY(launch_gemm<T>(g, pick_tile(g.M, g.N, g.part), st));
+        g.tail_split = true;
+        RMH_TRY(launch_gemm<T>(g, factor_tile(g.M, g.N, g.part, min_tiles), st));
         return h->prof_end(st, slot);
     };
     // recursive halving of [s0, s0+sw) down to leaves of at most `leafw`
@@ -506,7 +522,7 @@ rmh_status potrf_core(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
         }
         g.C = A + c0 + c0 * lda; g.ldc = lda;
         const double flops = 2.0 * kw * ((double)cols * (cols + 1) / 2.0 + (double)(rows - cols) * cols);
-        const Tile tile = pick_tile(g.M, g.N, g.part);
+        const Tile tile = factor_tile(g.M, g.N, g.part, min_tiles);
         g.one_per_cu = lookahead && share_cus && tile == Tile::T128x128;
         g.tail_split = true;
         int slot;

@@ -39,12 +39,15 @@ struct SView {
 // the latency-bound sweep, while the inverses grow as nbd^2 per row.  FP32
 // (the refinement solves of dsposv, three or four per call) takes 2048-row
 // blocks above n = 8192: 1.0-1.4 ms less per mixed solve at n = 16384; FP64
-// potrs, built and swept once, gains nothing from them.
+// potrs, built and swept once, gains nothing from them.  Below that, 1024-row
+// blocks from n = 2049 on and 512 from 1025 on (FP64 potrs with 64
+// right-hand sides: 0.90 -> 0.53 ms at n = 4096, 1.22 -> 0.96 ms at 8192,
+// 0.54 -> 0.41 ms at 2048 against 256 / 512 / 256-row blocks).
 template <typename T>
 int64_t choose_nbd(int64_t na) {
     if (na > 8192) return sizeof(T) == 4 ? 2048 : 1024;
-    if (na > 4096) return 512;
-    if (na > 1024) return 256;
+    if (na > 2048) return 1024;
+    if (na > 1024) return 512;
     return 128;
 }
 
