@@ -60,6 +60,9 @@ rmh_status build_block_inverses(const SView<T>& S, int64_t na, bool unit, int64_
         const int64_t ny = nbd / (2 * h);
         const int64_t full = na / (2 * h);  // pairs with both halves complete
         if (full > 0) {
+            // a full round of 128 x 128 tiles over the batch takes them (the
+            // top level at nbd = 2048); both tile shapes give the same sums
+            const Tile lt = (h / 128) * (h / 128) * full >= 512 ? Tile::T128x128 : Tile::T64x64;
             // W_p = Q_p inv(P_p)   (Q_p = S[r0+h : r0+2h, r0 : r0+h], r0 = 2h p)
             GemmDesc<T> g1;
             g1.M = h; g1.N = h; g1.K = h; g1.alpha = T(1); g1.beta = T(0);
@@ -71,7 +74,7 @@ rmh_status build_block_inverses(const SView<T>& S, int64_t na, bool unit, int64_
             g1.sAy = 2 * h * (1 + S.lda); g1.sAz = ny * g1.sAy;
             g1.sBy = 2 * h * (1 + na);    g1.sBz = ny * 2 * h;
             g1.sCy = h * h;               g1.sCz = ny * h * h;
-            RMH_TRY(launch_gemm<T>(g1, Tile::T64x64, s));
+            RMH_TRY(launch_gemm<T>(g1, lt, s));
             // X_p = -inv(R_p) W_p  into the strip below inv(P_p)
             GemmDesc<T> g2;
             g2.M = h; g2.N = h; g2.K = h; g2.alpha = T(-1); g2.beta = T(0);
@@ -82,7 +85,7 @@ rmh_status build_block_inverses(const SView<T>& S, int64_t na, bool unit, int64_
             g2.sAy = 2 * h * (1 + na); g2.sAz = ny * 2 * h;
             g2.sBy = h * h;            g2.sBz = ny * h * h;
             g2.sCy = 2 * h * (1 + na); g2.sCz = ny * 2 * h;
-            RMH_TRY(launch_gemm<T>(g2, Tile::T64x64, s));
+            RMH_TRY(launch_gemm<T>(g2, lt, s));
         }
         const int64_t r0 = full * 2 * h;
         if (na - r0 > h) {  // the last pair has a short lower half
