@@ -20,9 +20,12 @@
 // elements; the reference's row clears and the other triangle are replaced
 // by one final clear of that triangle.  The row swaps of the columns left of
 // the block (finished columns, not read while the block runs) are applied
-// after the block by a parallel kernel, in the same order.  L(i,j) of the lower view is A[i + j*lda] (lower) or
-// A[j + i*lda] (upper: the factor U = L^T, as the reference runs the lower
-// code on trans(A)).
+// after the block by a parallel kernel, in the same order, on a second
+// stream beside the next block.  Each thread owns fixed rows of the block
+// (k + tid + 1024 u) and keeps their running diagonal values; a column's
+// loads are all issued before its stores.  L(i,j) of the lower view is
+// A[i + j*lda] (lower) or A[j + i*lda] (upper: the factor U = L^T, as the
+// reference runs the lower code on trans(A)).
 #include <cfloat>
 
 #include "rmh_context.hpp"
@@ -83,123 +86,210 @@ __device__ __forceinline__ void argmax_combine(T& v, int& i, T ov, int oi) {
     }
 }
 
-// the columns k .. k+19 (fewer at the end) of the factorisation; the running
-// diagonal values live in LDS when lds_piv (n * sizeof(T) <= 64 KB), else in
-// the global workspace
+// lane s's value (s uniform) to the whole wavefront
 template <typename T>
-__global__ __launch_bounds__(kPsThreads) void pstrf_block_kernel(PsView<T> L, int64_t n, int64_t k, T* __restrict__ gpiv,
-                                                                 int64_t* __restrict__ P, int64_t* __restrict__ rank,
-                                                                 int64_t* __restrict__ done,
-                                                                 const double* __restrict__ eps_p, int lds_piv) {
-    extern __shared__ unsigned char ps_dyn[];
-    __shared__ T wv[kPsThreads / 64];
-    __shared__ int wi[kPsThreads / 64];
-    __shared__ int s_pivot;
-    __shared__ T rowc[kPsBlock];  // the pivot row in the block's earlier columns
-    if (*done != 0) return;
-    T* piv = lds_piv ? reinterpret_cast<T*>(ps_dyn) : gpiv;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const double eps = *eps_p;
-    const int cur = (int)min((int64_t)kPsBlock, n - k);
-    // this thread's candidate for the next column's pivot, from the running
-    // diagonal values it updated with the column it just computed
-    T nv = T(0);
-    int ni = -1;
-    for (int j = 0; j < cur; ++j) {
-        const int64_t c = k + j;
-        // running diagonal values: restarted at the block's first column,
-        // later columns' values were updated (and their candidates found)
-        // where the previous column was computed
-        T bv = nv;
-        int bi = ni;
-        if (j == 0) {
-            for (int64_t r = k + tid; r < n; r += kPsThreads) {
-                const T x = L(r, r);
-                piv[r] = x;
-                argmax_combine(bv, bi, x, (int)r);
-            }
-        }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const T ov = __shfl_xor(bv, off);
-            const int oi = __shfl_xor(bi, off);
-            argmax_combine(bv, bi, ov, oi);
-        }
-        if (lane == 0) {
-            wv[wave] = bv;
-            wi[wave] = bi;
-        }
-        __syncthreads();
-        if (wave == 0) {
-            T v = lane < kPsThreads / 64 ? wv[lane] : T(0);
-            int i = lane < kPsThreads / 64 ? wi[lane] : -1;
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const T ov = __shfl_xor(v, off);
-                const int oi = __shfl_xor(i, off);
-                argmax_combine(v, i, ov, oi);
-            }
-            if (lane == 0) s_pivot = i;
-        }
-        __syncthreads();
-        const int64_t p = s_pivot;
-        if (p != c) {  // p > c; the four pieces are disjoint
-            if (tid == 0) {
-                P[c] = p;
-                T x = L(c, c);
-                L(c, c) = L(p, p);
-                L(p, p) = x;
-                x = piv[c];
-                piv[c] = piv[p];
-                piv[p] = x;
-            }
-            // rows c and p in the block's own columns; the earlier blocks'
-            // columns take the block's swaps afterwards (pstrf_row_swaps_kernel)
-            for (int64_t t = k + tid; t < c; t += kPsThreads) {
-                const T x = L(c, t), y = L(p, t);
-                L(c, t) = y;
-                L(p, t) = x;
-                rowc[t - k] = y;
-            }
-            for (int64_t t = c + 1 + tid; t < p; t += kPsThreads) {
-                const T x = L(t, c);
-                L(t, c) = L(p, t);
-                L(p, t) = x;
-            }
-            for (int64_t t = p + 1 + tid; t < n; t += kPsThreads) {
-                const T x = L(t, c);
-                L(t, c) = L(t, p);
-                L(t, p) = x;
-            }
-        } else if (tid < j) {
-            rowc[tid] = L(c, k + tid);
-        }
-        __syncthreads();
-        const T pv = piv[c];
-        if ((double)pv < eps) {
-            if (tid == 0) {
-                *rank = c;
-                *done = 1;
-            }
-            return;
-        }
-        const T d = sqrt(pv);
-        if (tid == 0) L(c, c) = d;
-        // the block's earlier columns have not updated this one yet; the
-        // running diagonal value of each row takes the new entry at once
-        nv = T(0);
-        ni = -1;
-        for (int64_t r = c + 1 + tid; r < n; r += kPsThreads) {
-            T acc = T(0);
-            for (int t = 0; t < j; ++t) acc += L(r, k + t) * rowc[t];
-            const T y = (L(r, c) - acc) / d;
-            L(r, c) = y;
-            const T x = piv[r] - y * y;
-            piv[r] = x;
-            argmax_combine(nv, ni, x, (int)r);
-        }
-        __syncthreads();
+__device__ __forceinline__ T ps_lane(T v, int s) {
+    if constexpr (sizeof(T) == 8) {
+        const unsigned long long b = __double_as_longlong((double)v);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, s);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), s);
+        return (T)__longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+    } else {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int((float)v), s));
     }
+}
+
+constexpr int kPsWaves = kPsThreads / 64;
+// trailing rows per thread and pass (row k + tid + 1024 u): what the 128
+// registers of a 1024-thread workgroup hold without spilling
+template <typename T>
+constexpr int ps_rows() { return sizeof(T) == 8 ? 2 : 4; }
+
+// the (value, index) maximum of a wavefront, complete in lane 0
+template <typename T>
+__device__ __forceinline__ void ps_wave_max(T& v, int& i, int width) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        if (off >= width) continue;
+        const T ov = __shfl_xor(v, off);
+        const int oi = __shfl_xor(i, off);
+        argmax_combine(v, i, ov, oi);
+    }
+}
+
+// the lower view with its storage order fixed at compile time: L(i,j) is
+// A[i + j*ld] (lower) or A[j + i*ld] (upper)
+template <typename T, bool UP>
+struct PsViewS {
+    T* A;
+    int64_t ld;
+    __device__ T& operator()(int64_t i, int64_t j) const { return UP ? A[j + i * ld] : A[i + j * ld]; }
+    __device__ int64_t cs() const { return UP ? 1 : ld; }  // step between columns of a row
+};
+
+template <typename T, bool UP>
+struct PsBlock {
+    PsViewS<T, UP> L;
+    int64_t n, k;
+    T* gpiv;  // running diagonal values when they do not fit in LDS
+    int64_t *P, *rank, *done;
+    double eps;
+};
+
+// per-wavefront maxima (two buffers, by column parity) and, when they fit,
+// the running diagonal values: LDS variables, so that every access is an
+// LDS instruction
+template <typename T>
+__shared__ T ps_wv[2][kPsWaves];
+__shared__ int ps_wi[2][kPsWaves];
+extern __shared__ unsigned char ps_dyn[];
+
+template <typename T, bool LP>
+__device__ __forceinline__ T& ps_piv(const T* gpiv, int64_t i) {
+    if constexpr (LP) return reinterpret_cast<T*>(ps_dyn)[i];
+    else return const_cast<T*>(gpiv)[i];
+}
+
+// Column c = k + j of the block.  Every wavefront reduces the previous
+// column's per-wavefront maxima to the pivot p (the first largest running
+// diagonal value).  Then all reads come first:
+//   * lanes s < j hold row p and row c of the block's earlier columns (the
+//     new row c and, for the row that becomes p, its new entries);
+//   * each thread, for its rows t > c, reads the entry that the symmetric
+//     swap moves into column c (L(p,t) for t < p, L(t,p) for t > p, L(t,c)
+//     itself at t = p or without a swap), the old L(t,c) it moves out, and
+//     its j block-column entries (all j loads of a row issued together).
+// The thread then forms L(t,c) = (raw - sum_s L(t,k+s) L(c,k+s)) / d in the
+// order s = 0 .. j-1, updates its running value, and stores.  The block's
+// rows c and p are exchanged after a barrier (every wavefront has read
+// them), and a second barrier makes the column visible to the next one.
+// Returns false when the factorisation stops at this column.
+template <typename T, bool UP, bool LP>
+__device__ __forceinline__ bool pstrf_column(const PsBlock<T, UP>& b, int j, int tid) {
+    const PsViewS<T, UP>& L = b.L;
+    const int64_t n = b.n, k = b.k, c = k + j;
+    const int lane = tid & 63, wave = tid >> 6;
+    T v = lane < kPsWaves ? ps_wv<T>[j & 1][lane] : T(0);
+    int vi = lane < kPsWaves ? ps_wi[j & 1][lane] : -1;
+    ps_wave_max(v, vi, kPsWaves);
+    const int64_t p = __builtin_amdgcn_readlane(vi, 0);
+    const T pv = ps_lane(v, 0);
+    const bool sw = p != c;
+    T rp = T(0), rc = T(0);  // lane s < j: L(p, k+s) and L(c, k+s) before the swap
+    if (lane < j) {
+        rp = L(p, k + lane);
+        rc = L(c, k + lane);
+    }
+    if ((double)pv < b.eps) {
+        // the swap of rows c and p in the block's columns still applies; the
+        // rest of the swap lies in the square that is cleared
+        if (wave == 0 && sw && lane < j) {
+            L(c, k + lane) = rp;
+            L(p, k + lane) = rc;
+        }
+        if (tid == 0) {
+            b.P[c] = p;
+            *b.rank = c;
+            *b.done = 1;
+        }
+        return false;
+    }
+    const T d = sqrt(pv);
+    T xd = T(0);
+    if (tid == 0 && sw) xd = L(c, c);
+    T nv = T(0);  // this thread's candidate for the next pivot
+    int ni = -1;
+    const int64_t cs = L.cs();
+    for (int64_t q0 = k + tid; q0 < n; q0 += (int64_t)kPsThreads * ps_rows<T>()) {
+        T oc[ps_rows<T>()], raw[ps_rows<T>()], acc[ps_rows<T>()];
+#pragma unroll
+        for (int u = 0; u < ps_rows<T>(); ++u) {
+            const int64_t t = q0 + (int64_t)u * kPsThreads;
+            const int64_t r = t < n ? t : n - 1;  // rows past the end read a valid row and store nothing
+            oc[u] = L(r, c);
+            raw[u] = oc[u];
+            if (sw && r != p) raw[u] = r < p ? L(p, r) : L(r, p);
+            acc[u] = T(0);
+        }
+        // one row's block entries in flight at a time (the register budget
+        // of 1024 threads; at 4096 rows and j = 19 they are more than the
+        // register file, and the reads are bound by the L2)
+#pragma unroll
+        for (int u = 0; u < ps_rows<T>(); ++u) {
+            const int64_t t = q0 + (int64_t)u * kPsThreads;
+            const int64_t r = t < n ? t : n - 1;
+            const T* row = &L(sw && r == p ? c : r, k);
+            T a[kPsBlock - 1];
+#pragma unroll
+            for (int s = 0; s < kPsBlock - 1; ++s) a[s] = s < j ? row[s * cs] : T(0);
+#pragma unroll
+            for (int s = 0; s < kPsBlock - 1; ++s) {
+                if (s < j) acc[u] += a[s] * ps_lane(rp, s);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int u = 0; u < ps_rows<T>(); ++u) {
+            const int64_t t = q0 + (int64_t)u * kPsThreads;
+            if (!(t > c && t < n)) continue;
+            const T y = (raw[u] - acc[u]) / d;
+            if (sw && t != p) {
+                if (t < p) L(p, t) = oc[u];
+                else L(t, p) = oc[u];
+            }
+            L(t, c) = y;
+            const T x = ps_piv<T, LP>(b.gpiv, t == p ? c : t) - y * y;
+            ps_piv<T, LP>(b.gpiv, t) = x;
+            argmax_combine(nv, ni, x, (int)t);
+        }
+    }
+    ps_wave_max(nv, ni, 64);
+    if (lane == 0) {
+        ps_wv<T>[(j + 1) & 1][wave] = nv;
+        ps_wi[(j + 1) & 1][wave] = ni;
+    }
+    __syncthreads();
+    if (wave == 0 && sw && lane < j) {
+        L(c, k + lane) = rp;
+        L(p, k + lane) = rc;
+    }
+    if (tid == 0) {
+        b.P[c] = p;
+        if (sw) L(p, p) = xd;
+        L(c, c) = d;
+    }
+    __syncthreads();
+    return true;
+}
+
+// the columns k .. k+19 (fewer at the end) of the factorisation, one
+// workgroup; the running diagonal values live in LDS when lds_piv
+// (n * sizeof(T) <= 64 KB), else in the global workspace
+template <typename T, bool UP, bool LP>
+__global__ __launch_bounds__(kPsThreads) void pstrf_block_kernel(PsViewS<T, UP> L, int64_t n, int64_t k,
+                                                                 T* __restrict__ gpiv, int64_t* __restrict__ P,
+                                                                 int64_t* __restrict__ rank, int64_t* __restrict__ done,
+                                                                 const double* __restrict__ eps_p) {
+    if (*done != 0) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const PsBlock<T, UP> b{L, n, k, gpiv, P, rank, done, *eps_p};
+    // running diagonal values restart at the block's first column
+    T bv = T(0);
+    int bi = -1;
+    for (int64_t r = k + tid; r < n; r += kPsThreads) {
+        const T x = L(r, r);
+        ps_piv<T, LP>(gpiv, r) = x;
+        argmax_combine(bv, bi, x, (int)r);
+    }
+    ps_wave_max(bv, bi, 64);
+    if (lane == 0) {
+        ps_wv<T>[0][wave] = bv;
+        ps_wi[0][wave] = bi;
+    }
+    __syncthreads();
+    const int cur = (int)min((int64_t)kPsBlock, n - k);
+    for (int j = 0; j < cur; ++j)
+        if (!pstrf_column<T, UP, LP>(b, j, tid)) return;
 }
 
 // the row swaps of block k .. k+cur-1, in order, on the columns before k (not
@@ -312,14 +402,32 @@ rmh_status pstrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
     if (n == 0) return RMH_STATUS_SUCCESS;
     const size_t piv_bytes = (size_t)n * sizeof(T);
     const int lds_piv = piv_bytes <= 64 * 1024 ? 1 : 0;
+    // The row swaps of a block's columns to the left run on the handle's side
+    // stream: nothing later on the caller's stream reads or writes those
+    // columns before the final clear, which waits for the side stream.
+    hipStream_t side = nullptr;
+    hipEvent_t ev_block = nullptr, ev_swaps = nullptr;
+    if (n > kPsBlock) RMH_TRY(h->side_stream(&side, &ev_block, &ev_swaps));
     for (int64_t k = 0; k < n; k += kPsBlock) {
-        hipLaunchKernelGGL((pstrf_block_kernel<T>), dim3(1), dim3(kPsThreads), lds_piv ? piv_bytes : 0, s, L, n, k, piv,
-                           piv_out, rank, done, eps, lds_piv);
+        const size_t dyn = lds_piv ? piv_bytes : 0;
+#define PS_LAUNCH(UP, LP)                                                                                  \
+    hipLaunchKernelGGL((pstrf_block_kernel<T, UP, LP>), dim3(1), dim3(kPsThreads), dyn, s, PsViewS<T, UP>{A, lda}, n, \
+                       k, piv, piv_out, rank, done, eps)
+        if (upper) {
+            if (lds_piv) PS_LAUNCH(true, true);
+            else PS_LAUNCH(true, false);
+        } else {
+            if (lds_piv) PS_LAUNCH(false, true);
+            else PS_LAUNCH(false, false);
+        }
+#undef PS_LAUNCH
         RMH_TRY(check_launch("pstrf_block_kernel"));
         if (k > 0) {
             const int cur = (int)std::min<int64_t>(kPsBlock, n - k);
+            RMH_TRY_HIP(hipEventRecord(ev_block, s));
+            RMH_TRY_HIP(hipStreamWaitEvent(side, ev_block, 0));
             hipLaunchKernelGGL((pstrf_row_swaps_kernel<T>), dim3((unsigned)((k + kSwapCols - 1) / kSwapCols)),
-                               dim3(64, kSwapCols), 0, s, L, k, cur, piv_out);
+                               dim3(64, kSwapCols), 0, side, L, k, cur, piv_out);
             RMH_TRY(check_launch("pstrf_row_swaps_kernel"));
         }
         const int64_t m2 = n - k - kPsBlock;
@@ -341,6 +449,10 @@ rmh_status pstrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
         g.C = A + (k + kPsBlock) + (k + kPsBlock) * lda; g.ldc = lda;
         g.abort_flag = done;
         RMH_TRY(launch_gemm<T>(g, pick_tile(g.M, g.N, g.part), s));
+    }
+    if (side != nullptr) {
+        RMH_TRY_HIP(hipEventRecord(ev_swaps, side));
+        RMH_TRY_HIP(hipStreamWaitEvent(s, ev_swaps, 0));
     }
     const int64_t cells = n * n;
     const unsigned grid = (unsigned)std::min<int64_t>((cells + 255) / 256, 4096);
