@@ -12,20 +12,25 @@
 // loop (each column is O(n) work: pivot update, an argmax reduction, the
 // swap, a 20-term GEMV per row), the trailing update is the MFMA GEMM with
 // K = 20, and a device flag set at termination makes every later launch
-// return at once.  Only the factor's triangle is read and updated: the
-// symmetric row-and-column swap is applied to its pieces (rows c and p left
-// of c, the two diagonal entries, column c against row p between them, the
-// columns below p), which gives the same triangle as the reference's full
-// swaps of a symmetric matrix while touching a fraction of the strided
-// elements; the reference's row clears and the other triangle are replaced
-// by one final clear of that triangle.  The row swaps of the columns left of
-// the block (finished columns, not read while the block runs) are applied
-// after the block by a parallel kernel, in the same order, on a second
-// stream beside the next block.  Each thread owns fixed rows of the block
-// (k + tid + 1024 u) and keeps their running diagonal values; a column's
-// loads are all issued before its stores.  L(i,j) of the lower view is
-// A[i + j*lda] (lower) or A[j + i*lda] (upper: the factor U = L^T, as the
-// reference runs the lower code on trans(A)).
+// return at once.
+//
+// The matrix is first made symmetric in both triangles (the other triangle
+// is storage the caller does not read; it is cleared at the end, as the
+// reference's row clears leave it).  A block then reads A without writing
+// it: its row-and-column swaps are kept as a permutation, every raw column
+// is a contiguous physical column, and the block's columns go to a panel
+// buffer.  Two parallel kernels then put the panel and the (at most 20)
+// moved trailing rows and columns in place, and the trailing update covers
+// the whole square, as the reference's does.  The row swaps of the columns
+// left of the block (finished columns, not read while the block runs) are
+// applied after the block by a parallel kernel, in the same order, on a
+// second stream beside the next block.  Each thread owns fixed rows of the
+// block (k + tid + 1024 u) and keeps their running diagonal values; a
+// column's loads are all issued before its stores.
+//
+// Both storage orders run this lower algorithm: an upper factor U = L^T (the
+// reference runs the lower code on trans(A)) is the transpose of the lower
+// result, copied into the upper triangle at the end.
 #include <cfloat>
 
 #include "rmh_context.hpp"
@@ -117,57 +122,68 @@ __device__ __forceinline__ void ps_wave_max(T& v, int& i, int width) {
     }
 }
 
-// the lower view with its storage order fixed at compile time: L(i,j) is
-// A[i + j*ld] (lower) or A[j + i*ld] (upper)
-template <typename T, bool UP>
-struct PsViewS {
-    T* A;
-    int64_t ld;
-    __device__ T& operator()(int64_t i, int64_t j) const { return UP ? A[j + i * ld] : A[i + j * ld]; }
-    __device__ int64_t cs() const { return UP ? 1 : ld; }  // step between columns of a row
-};
+// ---- the block of 20 columns ----------------------------------------------
+// The matrix is kept symmetric in both triangles while the factorisation
+// runs, and a block reads it without writing it: the block's row-and-column
+// swaps are a permutation perm (logical row -> physical row of A, identity
+// at the block's start), so the column that the swap brings into position
+// c is physical column perm(p) read at rows perm(t) -- contiguous, where the
+// stored triangle alone would need row p, one strided line per entry.  The
+// block's columns go to a panel buffer Y by logical row; after the block,
+// pstrf_gather_kernel / pstrf_scatter_kernel move the (at most 40) trailing
+// rows and columns the permutation touched into place and write the panel,
+// and the trailing update runs on the whole square.  Same values, same
+// order of operations as swapping in place.
 
-template <typename T, bool UP>
-struct PsBlock {
-    PsViewS<T, UP> L;
+template <typename T>
+struct PsLazy {
+    T* A;
+    int64_t lda;
     int64_t n, k;
-    T* gpiv;  // running diagonal values when they do not fit in LDS
+    T* Y;  // the block's columns: L(k + r, k + s) = Y[r + s * n]
+    T* gpiv;  // running diagonal values (local row r = t - k) when not in LDS
+    int* gperm;  // the permutation (local row -> physical row); always exported here
     int64_t *P, *rank, *done;
+    int* info;  // [0] columns finished, [1] moved trailing rows (at most 20), [2..] their local indices
+    const double* eps_p;  // the stopping threshold (set by pstrf_init_kernel)
     double eps;
 };
 
 // per-wavefront maxima (two buffers, by column parity) and, when they fit,
-// the running diagonal values: LDS variables, so that every access is an
-// LDS instruction
+// the running diagonal values and the permutation: LDS variables, so that
+// every access is an LDS instruction
 template <typename T>
 __shared__ T ps_wv[2][kPsWaves];
 __shared__ int ps_wi[2][kPsWaves];
+__shared__ int ps_cnt;
 extern __shared__ unsigned char ps_dyn[];
 
 template <typename T, bool LP>
-__device__ __forceinline__ T& ps_piv(const T* gpiv, int64_t i) {
-    if constexpr (LP) return reinterpret_cast<T*>(ps_dyn)[i];
-    else return const_cast<T*>(gpiv)[i];
+__device__ __forceinline__ T& lz_piv(const PsLazy<T>& b, int64_t r) {
+    if constexpr (LP) return reinterpret_cast<T*>(ps_dyn)[r];
+    else return b.gpiv[r];
+}
+template <typename T, bool LP>
+__device__ __forceinline__ int& lz_perm(const PsLazy<T>& b, int64_t m, int64_t r) {
+    if constexpr (LP) return reinterpret_cast<int*>(reinterpret_cast<T*>(ps_dyn) + m)[r];
+    else return b.gperm[r];
 }
 
-// Column c = k + j of the block.  Every wavefront reduces the previous
-// column's per-wavefront maxima to the pivot p (the first largest running
-// diagonal value).  Then all reads come first:
-//   * lanes s < j hold row p and row c of the block's earlier columns (the
-//     new row c and, for the row that becomes p, its new entries);
-//   * each thread, for its rows t > c, reads the entry that the symmetric
-//     swap moves into column c (L(p,t) for t < p, L(t,p) for t > p, L(t,c)
-//     itself at t = p or without a swap), the old L(t,c) it moves out, and
-//     its j block-column entries (all j loads of a row issued together).
+// Column c = k + j of the block (local index j).  Every wavefront reduces
+// the previous column's per-wavefront maxima to the pivot p (the first
+// largest running diagonal value).  Then all reads come first:
+//   * lanes s < j hold rows p and c of the block's panel before the swap;
+//   * each thread, for its rows t > c, reads the entry that the swap brings
+//     into column c, A(perm(t'), perm(p)) with t' the row's position before
+//     the swap, and its j panel entries (row c's for the row that becomes p).
 // The thread then forms L(t,c) = (raw - sum_s L(t,k+s) L(c,k+s)) / d in the
-// order s = 0 .. j-1, updates its running value, and stores.  The block's
-// rows c and p are exchanged after a barrier (every wavefront has read
-// them), and a second barrier makes the column visible to the next one.
+// order s = 0 .. j-1 and updates its running value.  Rows c and p of the
+// panel and the permutation are exchanged after a barrier (every wavefront
+// has read them); a second barrier makes them visible to the next column.
 // Returns false when the factorisation stops at this column.
-template <typename T, bool UP, bool LP>
-__device__ __forceinline__ bool pstrf_column(const PsBlock<T, UP>& b, int j, int tid) {
-    const PsViewS<T, UP>& L = b.L;
-    const int64_t n = b.n, k = b.k, c = k + j;
+template <typename T, bool LP>
+__device__ __forceinline__ bool pstrf_column(const PsLazy<T>& b, int j, int tid) {
+    const int64_t n = b.n, k = b.k, m = n - k, c = k + j;
     const int lane = tid & 63, wave = tid >> 6;
     T v = lane < kPsWaves ? ps_wv<T>[j & 1][lane] : T(0);
     int vi = lane < kPsWaves ? ps_wi[j & 1][lane] : -1;
@@ -175,53 +191,52 @@ __device__ __forceinline__ bool pstrf_column(const PsBlock<T, UP>& b, int j, int
     const int64_t p = __builtin_amdgcn_readlane(vi, 0);
     const T pv = ps_lane(v, 0);
     const bool sw = p != c;
-    T rp = T(0), rc = T(0);  // lane s < j: L(p, k+s) and L(c, k+s) before the swap
+    const int64_t pl = p - k;  // local index of the pivot row
+    T* const Y = b.Y;
+    T rp = T(0), rc = T(0);  // lane s < j: panel rows p and c before the swap
     if (lane < j) {
-        rp = L(p, k + lane);
-        rc = L(c, k + lane);
+        rp = Y[pl + lane * n];
+        rc = Y[j + lane * n];
     }
     if ((double)pv < b.eps) {
         // the swap of rows c and p in the block's columns still applies; the
         // rest of the swap lies in the square that is cleared
         if (wave == 0 && sw && lane < j) {
-            L(c, k + lane) = rp;
-            L(p, k + lane) = rc;
+            Y[j + lane * n] = rp;
+            Y[pl + lane * n] = rc;
         }
         if (tid == 0) {
             b.P[c] = p;
             *b.rank = c;
             *b.done = 1;
+            b.info[0] = j;
         }
         return false;
     }
+    const int phys_c = lz_perm<T, LP>(b, m, j), phys_p = lz_perm<T, LP>(b, m, pl);
     const T d = sqrt(pv);
-    T xd = T(0);
-    if (tid == 0 && sw) xd = L(c, c);
+    const T* const colp = b.A + (int64_t)phys_p * b.lda;  // becomes column c
     T nv = T(0);  // this thread's candidate for the next pivot
     int ni = -1;
-    const int64_t cs = L.cs();
-    for (int64_t q0 = k + tid; q0 < n; q0 += (int64_t)kPsThreads * ps_rows<T>()) {
-        T oc[ps_rows<T>()], raw[ps_rows<T>()], acc[ps_rows<T>()];
+    for (int64_t q0 = tid; q0 < m; q0 += (int64_t)kPsThreads * ps_rows<T>()) {
+        T raw[ps_rows<T>()], acc[ps_rows<T>()];
 #pragma unroll
         for (int u = 0; u < ps_rows<T>(); ++u) {
-            const int64_t t = q0 + (int64_t)u * kPsThreads;
-            const int64_t r = t < n ? t : n - 1;  // rows past the end read a valid row and store nothing
-            oc[u] = L(r, c);
-            raw[u] = oc[u];
-            if (sw && r != p) raw[u] = r < p ? L(p, r) : L(r, p);
+            const int64_t r = q0 + (int64_t)u * kPsThreads;
+            const int64_t rl = r < m ? r : m - 1;  // rows past the end read a valid row and store nothing
+            raw[u] = colp[sw && rl == pl ? phys_c : lz_perm<T, LP>(b, m, rl)];
             acc[u] = T(0);
         }
-        // one row's block entries in flight at a time (the register budget
-        // of 1024 threads; at 4096 rows and j = 19 they are more than the
-        // register file, and the reads are bound by the L2)
+        // one row's panel entries in flight at a time (the register budget
+        // of 1024 threads)
 #pragma unroll
         for (int u = 0; u < ps_rows<T>(); ++u) {
-            const int64_t t = q0 + (int64_t)u * kPsThreads;
-            const int64_t r = t < n ? t : n - 1;
-            const T* row = &L(sw && r == p ? c : r, k);
+            const int64_t r = q0 + (int64_t)u * kPsThreads;
+            const int64_t rl = r < m ? r : m - 1;
+            const T* row = Y + (sw && rl == pl ? (int64_t)j : rl);
             T a[kPsBlock - 1];
 #pragma unroll
-            for (int s = 0; s < kPsBlock - 1; ++s) a[s] = s < j ? row[s * cs] : T(0);
+            for (int s = 0; s < kPsBlock - 1; ++s) a[s] = s < j ? row[s * n] : T(0);
 #pragma unroll
             for (int s = 0; s < kPsBlock - 1; ++s) {
                 if (s < j) acc[u] += a[s] * ps_lane(rp, s);
@@ -230,17 +245,13 @@ __device__ __forceinline__ bool pstrf_column(const PsBlock<T, UP>& b, int j, int
         }
 #pragma unroll
         for (int u = 0; u < ps_rows<T>(); ++u) {
-            const int64_t t = q0 + (int64_t)u * kPsThreads;
-            if (!(t > c && t < n)) continue;
+            const int64_t r = q0 + (int64_t)u * kPsThreads;
+            if (!(r > j && r < m)) continue;
             const T y = (raw[u] - acc[u]) / d;
-            if (sw && t != p) {
-                if (t < p) L(p, t) = oc[u];
-                else L(t, p) = oc[u];
-            }
-            L(t, c) = y;
-            const T x = ps_piv<T, LP>(b.gpiv, t == p ? c : t) - y * y;
-            ps_piv<T, LP>(b.gpiv, t) = x;
-            argmax_combine(nv, ni, x, (int)t);
+            Y[r + j * n] = y;
+            const T x = lz_piv<T, LP>(b, sw && r == pl ? (int64_t)j : r) - y * y;
+            lz_piv<T, LP>(b, r) = x;
+            argmax_combine(nv, ni, x, (int)(k + r));
         }
     }
     ps_wave_max(nv, ni, 64);
@@ -250,46 +261,128 @@ __device__ __forceinline__ bool pstrf_column(const PsBlock<T, UP>& b, int j, int
     }
     __syncthreads();
     if (wave == 0 && sw && lane < j) {
-        L(c, k + lane) = rp;
-        L(p, k + lane) = rc;
+        Y[j + lane * n] = rp;
+        Y[pl + lane * n] = rc;
     }
     if (tid == 0) {
         b.P[c] = p;
-        if (sw) L(p, p) = xd;
-        L(c, c) = d;
+        Y[j + j * n] = d;
+        if (sw) {
+            lz_perm<T, LP>(b, m, j) = phys_p;
+            lz_perm<T, LP>(b, m, pl) = phys_c;
+        }
     }
     __syncthreads();
     return true;
 }
 
 // the columns k .. k+19 (fewer at the end) of the factorisation, one
-// workgroup; the running diagonal values live in LDS when lds_piv
-// (n * sizeof(T) <= 64 KB), else in the global workspace
-template <typename T, bool UP, bool LP>
-__global__ __launch_bounds__(kPsThreads) void pstrf_block_kernel(PsViewS<T, UP> L, int64_t n, int64_t k,
-                                                                 T* __restrict__ gpiv, int64_t* __restrict__ P,
-                                                                 int64_t* __restrict__ rank, int64_t* __restrict__ done,
-                                                                 const double* __restrict__ eps_p) {
-    if (*done != 0) return;
+// workgroup; the running values and the permutation live in LDS when LP
+template <typename T, bool LP>
+__global__ __launch_bounds__(kPsThreads) void pstrf_block_kernel(PsLazy<T> b) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const PsBlock<T, UP> b{L, n, k, gpiv, P, rank, done, *eps_p};
-    // running diagonal values restart at the block's first column
+    if (*b.done != 0) {
+        if (tid == 0) b.info[0] = 0;
+        return;
+    }
+    const int64_t n = b.n, k = b.k, m = n - k;
+    b.eps = *b.eps_p;
+    // running diagonal values restart at the block's first column; the
+    // permutation starts as the identity
     T bv = T(0);
     int bi = -1;
-    for (int64_t r = k + tid; r < n; r += kPsThreads) {
-        const T x = L(r, r);
-        ps_piv<T, LP>(gpiv, r) = x;
-        argmax_combine(bv, bi, x, (int)r);
+    for (int64_t r = tid; r < m; r += kPsThreads) {
+        const T x = b.A[(k + r) + (k + r) * b.lda];
+        lz_piv<T, LP>(b, r) = x;
+        lz_perm<T, LP>(b, m, r) = (int)(k + r);
+        argmax_combine(bv, bi, x, (int)(k + r));
     }
     ps_wave_max(bv, bi, 64);
     if (lane == 0) {
         ps_wv<T>[0][wave] = bv;
         ps_wi[0][wave] = bi;
     }
+    if (tid == 0) ps_cnt = 0;
     __syncthreads();
-    const int cur = (int)min((int64_t)kPsBlock, n - k);
+    const int cur = (int)min((int64_t)kPsBlock, m);
     for (int j = 0; j < cur; ++j)
-        if (!pstrf_column<T, UP, LP>(b, j, tid)) return;
+        if (!pstrf_column<T, LP>(b, j, tid)) return;
+    // the permutation of the trailing rows, and the rows it moves
+    for (int64_t r = cur + tid; r < m; r += kPsThreads) {
+        const int pr = lz_perm<T, LP>(b, m, r);
+        if constexpr (LP) b.gperm[r] = pr;
+        if (pr != (int)(k + r)) b.info[2 + atomicAdd(&ps_cnt, 1)] = (int)r;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        b.info[0] = cur;
+        b.info[1] = ps_cnt;
+    }
+}
+
+// the trailing square's moved columns, gathered before anything is written:
+// column q of tmp (rows cur .. m-1 of the block, local) is logical column
+// info[2 + q], i.e. physical column perm(i) read at rows perm(r)
+template <typename T>
+__global__ __launch_bounds__(256) void pstrf_gather_kernel(const T* __restrict__ A, int64_t lda, int64_t n, int64_t k,
+                                                          int cur, const int* __restrict__ perm,
+                                                          const int* __restrict__ info, T* __restrict__ tmp) {
+    if (info[0] != cur) return;  // the factorisation stopped: the trailing square is cleared
+    const int q = blockIdx.y;
+    if (q >= info[1]) return;
+    const int64_t m = n - k, h = m - cur;
+    const T* col = A + (int64_t)perm[info[2 + q]] * lda;
+    for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < h; r += (int64_t)gridDim.x * 256)
+        tmp[r + q * h] = col[perm[cur + r]];
+}
+
+// the block's panel into its columns (lower part), and the gathered columns
+// into the trailing square as columns and as rows (the square stays
+// symmetric); blockIdx.y < cur: panel column, else moved column y - cur
+template <typename T>
+__global__ __launch_bounds__(256) void pstrf_scatter_kernel(T* __restrict__ A, int64_t lda, int64_t n, int64_t k, int cur,
+                                                           const T* __restrict__ Y, const int* __restrict__ info,
+                                                           const T* __restrict__ tmp) {
+    const int64_t m = n - k;
+    const int y = blockIdx.y;
+    if (y < cur) {
+        if (y >= info[0]) return;  // columns past a stop
+        for (int64_t r = y + (int64_t)blockIdx.x * 256 + threadIdx.x; r < m; r += (int64_t)gridDim.x * 256)
+            A[(k + r) + (k + y) * lda] = Y[r + (int64_t)y * n];
+        return;
+    }
+    if (info[0] != cur) return;
+    const int q = y - cur;
+    if (q >= info[1]) return;
+    const int64_t h = m - cur, i = k + info[2 + q];
+    for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < h; r += (int64_t)gridDim.x * 256) {
+        const T x = tmp[r + q * h];
+        A[(k + cur + r) + i * lda] = x;
+        A[i + (k + cur + r) * lda] = x;
+    }
+}
+
+// one triangle copied onto the other (32 x 32 tiles through LDS): from the
+// lower (from_lower) or the upper triangle, strictly off the diagonal
+template <typename T>
+__global__ __launch_bounds__(256) void pstrf_mirror_kernel(T* __restrict__ A, int64_t lda, int64_t n, bool from_lower) {
+    __shared__ T tile[32][33];
+    const int64_t t = blockIdx.x;
+    int64_t bi = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while (bi * (bi + 1) / 2 > t) --bi;
+    while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
+    const int64_t bj = t - bi * (bi + 1) / 2;  // lower tile (bi, bj), bi >= bj
+    const int64_t R0 = 32 * (from_lower ? bi : bj), C0 = 32 * (from_lower ? bj : bi);
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int yy = ty; yy < 32; yy += 8) {
+        const int64_t r = R0 + tx, c = C0 + yy;
+        if (r < n && c < n) tile[yy][tx] = A[r + c * lda];
+    }
+    __syncthreads();
+    for (int yy = ty; yy < 32; yy += 8) {
+        const int64_t sr = R0 + yy, sc = C0 + tx;  // source of destination (sc, sr)
+        if (sr < n && sc < n && (from_lower ? sr > sc : sr < sc)) A[sc + sr * lda] = tile[tx][yy];
+    }
 }
 
 // the row swaps of block k .. k+cur-1, in order, on the columns before k (not
@@ -390,69 +483,87 @@ template <typename T>
 rmh_status pstrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int64_t* piv_out, int64_t* rank,
                       hipStream_t s) {
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t nn = (size_t)std::max<int64_t>(n, 1);
+    const size_t b_piv = al(nn * sizeof(T)), b_perm = al(nn * sizeof(int)), b_pan = al(nn * kPsBlock * sizeof(T));
     void* base = nullptr;
-    RMH_TRY(h->reserve_slot(WS_PSTRF, al((size_t)std::max<int64_t>(n, 1) * sizeof(T)) + 512, &base));
-    T* piv = static_cast<T*>(base);
-    char* tail = static_cast<char*>(base) + al((size_t)std::max<int64_t>(n, 1) * sizeof(T));
+    RMH_TRY(h->reserve_slot(WS_PSTRF, b_piv + b_perm + 2 * b_pan + 1024, &base));
+    char* w = static_cast<char*>(base);
+    T* gpiv = reinterpret_cast<T*>(w);
+    int* gperm = reinterpret_cast<int*>(w + b_piv);
+    T* Y = reinterpret_cast<T*>(w + b_piv + b_perm);
+    T* tmp = reinterpret_cast<T*>(w + b_piv + b_perm + b_pan);
+    char* tail = w + b_piv + b_perm + 2 * b_pan;
     double* eps = reinterpret_cast<double*>(tail);
     int64_t* done = reinterpret_cast<int64_t*>(tail + 256);
-    const PsView<T> L{A, upper ? lda : 1, upper ? 1 : lda};
+    int* info = reinterpret_cast<int*>(tail + 512);
+    // both storage orders run the lower algorithm on a symmetric working copy
+    // (the other triangle is the caller's don't-care storage, cleared at the
+    // end); an upper factor is the transpose of the lower one
+    const PsView<T> L{A, 1, lda};
     hipLaunchKernelGGL((pstrf_init_kernel<T>), dim3(1), dim3(kPsThreads), 0, s, L, n, piv_out, rank, done, eps);
     RMH_TRY(check_launch("pstrf_init_kernel"));
     if (n == 0) return RMH_STATUS_SUCCESS;
-    const size_t piv_bytes = (size_t)n * sizeof(T);
-    const int lds_piv = piv_bytes <= 64 * 1024 ? 1 : 0;
+    const int64_t nt = (n + 31) / 32;
+    const unsigned mirror_tiles = (unsigned)(nt * (nt + 1) / 2);
+    hipLaunchKernelGGL((pstrf_mirror_kernel<T>), dim3(mirror_tiles), dim3(256), 0, s, A, lda, n, !upper);
+    RMH_TRY(check_launch("pstrf_mirror_kernel"));
     // The row swaps of a block's columns to the left run on the handle's side
     // stream: nothing later on the caller's stream reads or writes those
     // columns before the final clear, which waits for the side stream.
     hipStream_t side = nullptr;
     hipEvent_t ev_block = nullptr, ev_swaps = nullptr;
     if (n > kPsBlock) RMH_TRY(h->side_stream(&side, &ev_block, &ev_swaps));
+    constexpr size_t kLdsBudget = 96 * 1024;
     for (int64_t k = 0; k < n; k += kPsBlock) {
-        const size_t dyn = lds_piv ? piv_bytes : 0;
-#define PS_LAUNCH(UP, LP)                                                                                  \
-    hipLaunchKernelGGL((pstrf_block_kernel<T, UP, LP>), dim3(1), dim3(kPsThreads), dyn, s, PsViewS<T, UP>{A, lda}, n, \
-                       k, piv, piv_out, rank, done, eps)
-        if (upper) {
-            if (lds_piv) PS_LAUNCH(true, true);
-            else PS_LAUNCH(true, false);
-        } else {
-            if (lds_piv) PS_LAUNCH(false, true);
-            else PS_LAUNCH(false, false);
-        }
-#undef PS_LAUNCH
+        const int64_t m = n - k;
+        const int cur = (int)std::min<int64_t>(kPsBlock, m);
+        const bool lp = (size_t)m * (sizeof(T) + sizeof(int)) <= kLdsBudget;
+        const PsLazy<T> b{A, lda, n, k, Y, gpiv, gperm, piv_out, rank, done, info, eps, 0.0};
+        if (lp)
+            hipLaunchKernelGGL((pstrf_block_kernel<T, true>), dim3(1), dim3(kPsThreads),
+                               (size_t)m * (sizeof(T) + sizeof(int)), s, b);
+        else
+            hipLaunchKernelGGL((pstrf_block_kernel<T, false>), dim3(1), dim3(kPsThreads), 0, s, b);
         RMH_TRY(check_launch("pstrf_block_kernel"));
+        const int64_t h2 = m - cur;
+        if (h2 > 0) {
+            const unsigned gx = (unsigned)std::min<int64_t>((h2 + 255) / 256, 64);
+            hipLaunchKernelGGL((pstrf_gather_kernel<T>), dim3(gx, kPsBlock), dim3(256), 0, s, A, lda, n, k, cur, gperm,
+                               info, tmp);
+            RMH_TRY(check_launch("pstrf_gather_kernel"));
+        }
+        const unsigned sx = (unsigned)std::min<int64_t>((m + 255) / 256, 64);
+        hipLaunchKernelGGL((pstrf_scatter_kernel<T>), dim3(sx, (unsigned)(cur + (h2 > 0 ? kPsBlock : 0))), dim3(256), 0,
+                           s, A, lda, n, k, cur, Y, info, tmp);
+        RMH_TRY(check_launch("pstrf_scatter_kernel"));
         if (k > 0) {
-            const int cur = (int)std::min<int64_t>(kPsBlock, n - k);
             RMH_TRY_HIP(hipEventRecord(ev_block, s));
             RMH_TRY_HIP(hipStreamWaitEvent(side, ev_block, 0));
             hipLaunchKernelGGL((pstrf_row_swaps_kernel<T>), dim3((unsigned)((k + kSwapCols - 1) / kSwapCols)),
                                dim3(64, kSwapCols), 0, side, L, k, cur, piv_out);
             RMH_TRY(check_launch("pstrf_row_swaps_kernel"));
         }
-        const int64_t m2 = n - k - kPsBlock;
-        if (m2 <= 0) continue;
-        // trailing triangle -= L21 L21^T (the reference updates the whole square;
-        // only the factor's triangle is read here)
+        if (h2 <= 0) continue;
+        // trailing square -= L21 L21^T, both triangles (the reference updates
+        // the whole square too), so that the next block reads columns only
         GemmDesc<T> g;
-        g.part = upper ? Part::Upper : Part::Lower;
-        g.M = m2; g.N = m2; g.K = kPsBlock;
+        g.part = Part::Full;
+        g.M = h2; g.N = h2; g.K = cur;
         g.alpha = T(-1); g.beta = T(1);
-        if (!upper) {  // L21 = A[k+20:, k:k+20]
-            g.A = A + (k + kPsBlock) + k * lda; g.transA = false;
-            g.B = g.A; g.transB = true;
-        } else {       // L21 = S^T, S = A[k:k+20, k+20:]
-            g.A = A + k + (k + kPsBlock) * lda; g.transA = true;
-            g.B = g.A; g.transB = false;
-        }
+        g.A = A + (k + cur) + k * lda; g.transA = false;
+        g.B = g.A; g.transB = true;
         g.lda = lda; g.ldb = lda;
-        g.C = A + (k + kPsBlock) + (k + kPsBlock) * lda; g.ldc = lda;
+        g.C = A + (k + cur) + (k + cur) * lda; g.ldc = lda;
         g.abort_flag = done;
         RMH_TRY(launch_gemm<T>(g, pick_tile(g.M, g.N, g.part), s));
     }
     if (side != nullptr) {
         RMH_TRY_HIP(hipEventRecord(ev_swaps, side));
         RMH_TRY_HIP(hipStreamWaitEvent(s, ev_swaps, 0));
+    }
+    if (upper) {
+        hipLaunchKernelGGL((pstrf_mirror_kernel<T>), dim3(mirror_tiles), dim3(256), 0, s, A, lda, n, true);
+        RMH_TRY(check_launch("pstrf_mirror_kernel"));
     }
     const int64_t cells = n * n;
     const unsigned grid = (unsigned)std::min<int64_t>((cells + 255) / 256, 4096);
