@@ -454,7 +454,10 @@ __device__ __attribute__((noinline)) void solve_one(const T* A, int n, int64_t l
 // The persistent loop.  FP64 calls the out-of-line routines (about 390
 // registers, one workgroup per CU).  FP32 inlines them into a kernel held
 // to 256 registers, two workgroups per CU: a few spills, but 10.5 against
-// 13.3 ms on the 4096-system benchmark, where FP64 loses 7% the same way.
+// 13.3 ms on the 4096-system benchmark.  FP64 cannot run two per CU with
+// the deferred update (its two panel buffers take 139 KB of LDS); the
+// inlined NB 16 kernel without it (74 KB, 115 registers spilled) took
+// 11.1 ms against 8.5 ms for the deferred one per CU.
 template <typename T, int NB, bool INL>
 __device__ __forceinline__ void batched_posv_body(const BArgs<T>& a) {
     Smem<T, NB>& s = g_bsm<T, NB>;
