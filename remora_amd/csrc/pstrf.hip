@@ -228,11 +228,13 @@ __device__ __forceinline__ bool pstrf_column(const PsLazy<T>& b, int j, int tid)
             acc[u] = T(0);
         }
         // one row's panel entries in flight at a time (the register budget
-        // of 1024 threads)
+        // of 1024 threads); rows past the end skip theirs (whole wavefronts
+        // but one), each batch being a dependent round trip
 #pragma unroll
         for (int u = 0; u < ps_rows<T>(); ++u) {
             const int64_t r = q0 + (int64_t)u * kPsThreads;
-            const int64_t rl = r < m ? r : m - 1;
+            if (r >= m) continue;
+            const int64_t rl = r;
             const T* row = Y + (sw && rl == pl ? (int64_t)j : rl);
             T a[kPsBlock - 1];
 #pragma unroll
