@@ -105,10 +105,14 @@ __device__ __forceinline__ T ps_lane(T v, int s) {
 }
 
 constexpr int kPsWaves = kPsThreads / 64;
-// trailing rows per thread and pass (row k + tid + 1024 u): what the 128
-// registers of a 1024-thread workgroup hold without spilling
-template <typename T>
-constexpr int ps_rows() { return sizeof(T) == 8 ? 2 : 4; }
+// Trailing rows per thread and pass (row k + tid + 1024 u), R: 4 fit the
+// 128 registers of a 1024-thread workgroup without spilling.  Each pass
+// costs a dependent round trip for its swapped-in column, each row one for
+// its panel entries (rows past the end skip theirs).  In FP64, blocks with
+// more than 2048 trailing rows take four rows in one pass (n = 4096:
+// 34.75 -> 34.27 ms), smaller ones two (n = 2000: 11.80 against 12.04 ms
+// with four).
+constexpr int kPsRowsMax = 4;
 
 // the (value, index) maximum of a wavefront, complete in lane 0
 template <typename T>
@@ -130,7 +134,7 @@ __device__ __forceinline__ void ps_wave_max(T& v, int& i, int width) {
 // c is physical column perm(p) read at rows perm(t) -- contiguous, where the
 // stored triangle alone would need row p, one strided line per entry.  The
 // block's columns go to a panel buffer Y by logical row; after the block,
-// pstrf_gather_kernel / pstrf_scatter_kernel move the (at most 40) trailing
+// pstrf_gather_kernel / pstrf_scatter_kernel move the (at most 20) trailing
 // rows and columns the permutation touched into place and write the panel,
 // and the trailing update runs on the whole square.  Same values, same
 // order of operations as swapping in place.
@@ -181,7 +185,7 @@ __device__ __forceinline__ int& lz_perm(const PsLazy<T>& b, int64_t m, int64_t r
 // panel and the permutation are exchanged after a barrier (every wavefront
 // has read them); a second barrier makes them visible to the next column.
 // Returns false when the factorisation stops at this column.
-template <typename T, bool LP>
+template <typename T, bool LP, int R>
 __device__ __forceinline__ bool pstrf_column(const PsLazy<T>& b, int j, int tid) {
     const int64_t n = b.n, k = b.k, m = n - k, c = k + j;
     const int lane = tid & 63, wave = tid >> 6;
@@ -218,10 +222,10 @@ __device__ __forceinline__ bool pstrf_column(const PsLazy<T>& b, int j, int tid)
     const T* const colp = b.A + (int64_t)phys_p * b.lda;  // becomes column c
     T nv = T(0);  // this thread's candidate for the next pivot
     int ni = -1;
-    for (int64_t q0 = tid; q0 < m; q0 += (int64_t)kPsThreads * ps_rows<T>()) {
-        T raw[ps_rows<T>()], acc[ps_rows<T>()];
+    for (int64_t q0 = tid; q0 < m; q0 += (int64_t)kPsThreads * R) {
+        T raw[R], acc[R];
 #pragma unroll
-        for (int u = 0; u < ps_rows<T>(); ++u) {
+        for (int u = 0; u < R; ++u) {
             const int64_t r = q0 + (int64_t)u * kPsThreads;
             const int64_t rl = r < m ? r : m - 1;  // rows past the end read a valid row and store nothing
             raw[u] = colp[sw && rl == pl ? phys_c : lz_perm<T, LP>(b, m, rl)];
@@ -231,7 +235,7 @@ __device__ __forceinline__ bool pstrf_column(const PsLazy<T>& b, int j, int tid)
         // of 1024 threads); rows past the end skip theirs (whole wavefronts
         // but one), each batch being a dependent round trip
 #pragma unroll
-        for (int u = 0; u < ps_rows<T>(); ++u) {
+        for (int u = 0; u < R; ++u) {
             const int64_t r = q0 + (int64_t)u * kPsThreads;
             if (r >= m) continue;
             const int64_t rl = r;
@@ -246,7 +250,7 @@ __device__ __forceinline__ bool pstrf_column(const PsLazy<T>& b, int j, int tid)
             __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
-        for (int u = 0; u < ps_rows<T>(); ++u) {
+        for (int u = 0; u < R; ++u) {
             const int64_t r = q0 + (int64_t)u * kPsThreads;
             if (!(r > j && r < m)) continue;
             const T y = (raw[u] - acc[u]) / d;
@@ -280,7 +284,7 @@ __device__ __forceinline__ bool pstrf_column(const PsLazy<T>& b, int j, int tid)
 
 // the columns k .. k+19 (fewer at the end) of the factorisation, one
 // workgroup; the running values and the permutation live in LDS when LP
-template <typename T, bool LP>
+template <typename T, bool LP, int R>
 __global__ __launch_bounds__(kPsThreads) void pstrf_block_kernel(PsLazy<T> b) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (*b.done != 0) {
@@ -308,7 +312,7 @@ __global__ __launch_bounds__(kPsThreads) void pstrf_block_kernel(PsLazy<T> b) {
     __syncthreads();
     const int cur = (int)min((int64_t)kPsBlock, m);
     for (int j = 0; j < cur; ++j)
-        if (!pstrf_column<T, LP>(b, j, tid)) return;
+        if (!pstrf_column<T, LP, R>(b, j, tid)) return;
     // the permutation of the trailing rows, and the rows it moves
     for (int64_t r = cur + tid; r < m; r += kPsThreads) {
         const int pr = lz_perm<T, LP>(b, m, r);
@@ -521,11 +525,16 @@ rmh_status pstrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
         const int cur = (int)std::min<int64_t>(kPsBlock, m);
         const bool lp = (size_t)m * (sizeof(T) + sizeof(int)) <= kLdsBudget;
         const PsLazy<T> b{A, lda, n, k, Y, gpiv, gperm, piv_out, rank, done, info, eps, 0.0};
-        if (lp)
-            hipLaunchKernelGGL((pstrf_block_kernel<T, true>), dim3(1), dim3(kPsThreads),
-                               (size_t)m * (sizeof(T) + sizeof(int)), s, b);
+        const size_t dyn = lp ? (size_t)m * (sizeof(T) + sizeof(int)) : 0;
+        const bool four = sizeof(T) == 4 || m > 2 * kPsThreads;
+        if (lp && four)
+            hipLaunchKernelGGL((pstrf_block_kernel<T, true, kPsRowsMax>), dim3(1), dim3(kPsThreads), dyn, s, b);
+        else if (lp)
+            hipLaunchKernelGGL((pstrf_block_kernel<T, true, 2>), dim3(1), dim3(kPsThreads), dyn, s, b);
+        else if (four)
+            hipLaunchKernelGGL((pstrf_block_kernel<T, false, kPsRowsMax>), dim3(1), dim3(kPsThreads), 0, s, b);
         else
-            hipLaunchKernelGGL((pstrf_block_kernel<T, false>), dim3(1), dim3(kPsThreads), 0, s, b);
+            hipLaunchKernelGGL((pstrf_block_kernel<T, false, 2>), dim3(1), dim3(kPsThreads), 0, s, b);
         RMH_TRY(check_launch("pstrf_block_kernel"));
         const int64_t h2 = m - cur;
         if (h2 > 0) {
