@@ -164,6 +164,8 @@ Context::~Context() {
         (void)hipStreamSynchronize(trail);
         (void)hipStreamDestroy(trail);
     }
+    if (ev_conv != nullptr) (void)hipEventDestroy(ev_conv);
+    if (ev_conv0 != nullptr) (void)hipEventDestroy(ev_conv0);
     if (ev_c != nullptr) {
         (void)hipEventDestroy(ev_c);
         (void)hipEventDestroy(ev_d);
@@ -396,9 +398,7 @@ template <typename T>
 rmh_status potrf_core(Context* h, bool upper, int64_t n, T* A, int64_t lda, int64_t* d_info, void* ws, bool plain,
                       hipStream_t s) {
     const bool bf16x3 = sizeof(T) == 4 && (h->bf16x3_trailing || h->spotrf_bf16x3);
-    const int64_t nb = (int64_t)(h->nb != 0 ? h->nb
-                                            : (sizeof(T) == 8 ? Context::kDefaultNbF64
-                                                              : (bf16x3 ? Context::kDefaultNbBf16x3 : Context::kDefaultNbF32)));
+    const int64_t nb = potrf_panel_width(h, sizeof(T) == 8, bf16x3);
     const int64_t nbi = (int64_t)h->nbi;
     const int64_t min_tiles = bf16x3 ? 256 : 0;  // factor_tile
     // look-ahead mode: 0 single stream; 1 look-ahead; 2 with one trailing
@@ -562,6 +562,7 @@ rmh_status potrf_core(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
         return RMH_STATUS_SUCCESS;
     };
     RMH_TRY(factor_panel(0, std::min(nb, n), s));
+    if (h->cols_ready != nullptr) RMH_TRY_HIP(hipStreamWaitEvent(s, h->cols_ready, 0));
     for (int64_t k0 = 0; k0 < n; k0 += nb) {
         const int64_t kw = std::min(nb, n - k0);
         const int64_t k1 = k0 + kw;  // next panel

@@ -26,17 +26,40 @@ constexpr int kT = 64;  // conversion tile
 // into one slot.  Every slot (s, r), s < number of blocks, has exactly one
 // writer, so rowsum_kernel adds them in a fixed order: no atomics, and the
 // same ||A||_inf on every run.
+// the lower-tile enumeration: tile t of a triangle as (ti, tj), ti >= tj
+__device__ __forceinline__ void tri_tile64(int64_t t, int64_t& ti, int64_t& tj) {
+    ti = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while (ti * (ti + 1) / 2 > t) --ti;
+    while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+    tj = t - ti * (ti + 1) / 2;
+}
+
+// part 0: every tile; part 1: the tiles with tj < pt (the first panel of
+// the factorisation, pt tile columns); part 2: the others
 __global__ __launch_bounds__(256) void convert_sym_kernel(const double* __restrict__ A, int64_t lda,
                                                           float* __restrict__ A32, int64_t ld32, int64_t n,
-                                                          bool upper, double* __restrict__ rsp) {
+                                                          bool upper, double* __restrict__ rsp, int part,
+                                                          int64_t pt) {
     __shared__ double mag[kT][kT + 1];  // |a| of the stored off-diagonal entries, [row][col]
     __shared__ double rpart[4][kT], cpart[4][kT];
     // tile (ti, tj), ti >= tj, in the lower-tile enumeration
     const int64_t t = blockIdx.x;
-    int64_t ti = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-    while (ti * (ti + 1) / 2 > t) --ti;
-    while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
-    const int64_t tj = t - ti * (ti + 1) / 2;
+    int64_t ti, tj;
+    if (part == 1) {
+        const int64_t tri = pt * (pt + 1) / 2;
+        if (t < tri) {
+            tri_tile64(t, ti, tj);
+        } else {
+            ti = pt + (t - tri) / pt;
+            tj = (t - tri) % pt;
+        }
+    } else {
+        tri_tile64(t, ti, tj);
+        if (part == 2) {
+            ti += pt;
+            tj += pt;
+        }
+    }
     // lower storage: rows ti, cols tj.  upper storage: rows tj, cols ti.
     const int64_t rb = upper ? tj : ti, cb = upper ? ti : tj;
     const int64_t r0 = rb * kT, c0 = cb * kT;
@@ -254,13 +277,41 @@ rmh_status dsposv_impl(Context* h, bool upper, rmh_factor_precision prec, int64_
     RMH_TRY(mixed_ws(h, n, nrhs, &w));
     // 1. FP32 copy of the stored triangle and ||A||_inf
     const int64_t nt = (n + kT - 1) / kT;
-    hipLaunchKernelGGL(convert_sym_kernel, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), 0, s, A, lda, w.A32, n, n,
-                       upper, w.rsp);
-    RMH_TRY(check_launch("convert_sym_kernel"));
-    hipLaunchKernelGGL(rowsum_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w.rsp, n, nt, w.rowsum);
+    // The first panel's tiles are converted on s; the rest, the row sums and
+    // the norm run on the side stream beside the first panel's
+    // factorisation, which waits for them before its first trailing update
+    // (h->cols_ready).  The conversion is bandwidth-bound, the first panel's
+    // chain latency-bound.
+    const int64_t pt = std::min<int64_t>(nt, (potrf_panel_width(h, false, prec == RMH_FACTOR_BF16X3) + kT - 1) / kT);
+    hipStream_t s2 = nullptr;
+    hipEvent_t side_a = nullptr, side_b = nullptr;
+    const bool overlap = pt < nt;
+    if (overlap) {
+        RMH_TRY(h->side_stream(&s2, &side_a, &side_b));
+        if (h->ev_conv == nullptr) RMH_TRY_HIP(hipEventCreateWithFlags(&h->ev_conv, hipEventDisableTiming));
+        if (h->ev_conv0 == nullptr) RMH_TRY_HIP(hipEventCreateWithFlags(&h->ev_conv0, hipEventDisableTiming));
+        RMH_TRY_HIP(hipEventRecord(h->ev_conv0, s));  // the caller's earlier work on A
+        RMH_TRY_HIP(hipStreamWaitEvent(s2, h->ev_conv0, 0));
+        const int64_t rest = nt - pt;
+        hipLaunchKernelGGL(convert_sym_kernel, dim3((unsigned)(rest * (rest + 1) / 2)), dim3(256), 0, s2, A, lda, w.A32,
+                           n, n, upper, w.rsp, 2, pt);
+        RMH_TRY(check_launch("convert_sym_kernel"));
+        hipLaunchKernelGGL(convert_sym_kernel, dim3((unsigned)(pt * (pt + 1) / 2 + rest * pt)), dim3(256), 0, s, A, lda,
+                           w.A32, n, n, upper, w.rsp, 1, pt);
+        RMH_TRY(check_launch("convert_sym_kernel"));
+        RMH_TRY_HIP(hipEventRecord(h->ev_conv0, s));  // the first panel's partial row sums
+        RMH_TRY_HIP(hipStreamWaitEvent(s2, h->ev_conv0, 0));
+    } else {
+        hipLaunchKernelGGL(convert_sym_kernel, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), 0, s, A, lda, w.A32, n,
+                           n, upper, w.rsp, 0, pt);
+        RMH_TRY(check_launch("convert_sym_kernel"));
+    }
+    hipStream_t sn = overlap ? s2 : s;
+    hipLaunchKernelGGL(rowsum_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, sn, w.rsp, n, nt, w.rowsum);
     RMH_TRY(check_launch("rowsum_kernel"));
-    hipLaunchKernelGGL(max_kernel, dim3(1), dim3(256), 0, s, w.rowsum, n, w.anorm);
+    hipLaunchKernelGGL(max_kernel, dim3(1), dim3(256), 0, sn, w.rowsum, n, w.anorm);
     RMH_TRY(check_launch("max_kernel"));
+    if (overlap) RMH_TRY_HIP(hipEventRecord(h->ev_conv, s2));
     // 2. factor in reduced precision (FP64 diagonal blocks, FP32 MFMA updates)
     {
         struct Bf16x3 {  // the split-BF16x3 trailing updates for this factorisation only
@@ -275,8 +326,15 @@ rmh_status dsposv_impl(Context* h, bool upper, rmh_factor_precision prec, int64_
                 h->spotrf_bf16x3 = saved;
             }
         } mode(h, prec == RMH_FACTOR_BF16X3);
+        struct ColsReady {  // the factorisation's wait for the side stream's conversion
+            Context* h;
+            ColsReady(Context* c, hipEvent_t e) : h(c) { h->cols_ready = e; }
+            ~ColsReady() { h->cols_ready = nullptr; }
+        } ready(h, overlap ? h->ev_conv : nullptr);
         RMH_TRY(potrf_impl<float>(h, upper, n, w.A32, n, w.info, s));
     }
+    // a factorisation of one panel does not reach its wait
+    if (overlap) RMH_TRY_HIP(hipStreamWaitEvent(s, h->ev_conv, 0));
     int64_t info = 0;
     RMH_TRY_HIP(hipMemcpyAsync(&info, w.info, sizeof(info), hipMemcpyDeviceToHost, s));
     RMH_TRY_HIP(hipStreamSynchronize(s));

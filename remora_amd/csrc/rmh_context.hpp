@@ -48,6 +48,11 @@ struct Context {
     bool spotrf_bf16x3 = false;  // rmh_set_spotrf_precision
     hipStream_t side = nullptr;  // look-ahead stream of potrf (default priority)
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    // set by rmh_dsposv around its factorisation: the columns right of the
+    // first panel are ready (converted on the side stream) once this event
+    // has completed; potrf waits for it after factoring the first panel
+    hipEvent_t cols_ready = nullptr;
+    hipEvent_t ev_conv = nullptr, ev_conv0 = nullptr;  // rmh_dsposv's own events (made on first use)
     // look-ahead potrf: compute units per XCD kept free of the trailing
     // updates (rmh_set_potrf_reserve), so that the panel launches of the side
     // stream find room at once; the updates then run on `trail`, a stream
@@ -74,6 +79,13 @@ struct Context {
     rmh_status batch_counters(unsigned** c);
     ~Context();
 };
+
+// panel width of a factorisation: the handle's setting, else the default of
+// its precision (FP64, FP32, or FP32 with split-BF16x3 trailing updates)
+inline int64_t potrf_panel_width(const Context* h, bool fp64, bool bf16x3) {
+    if (h->nb != 0) return (int64_t)h->nb;
+    return (int64_t)(fp64 ? Context::kDefaultNbF64 : (bf16x3 ? Context::kDefaultNbBf16x3 : Context::kDefaultNbF32));
+}
 
 template <typename T>
 rmh_status trsm_impl(Context* h, bool left, bool upper, bool trans, bool unit, int64_t m, int64_t n, T alpha,
