@@ -167,6 +167,13 @@ __device__ __forceinline__ T& lz_piv(const PsLazy<T>& b, int64_t r) {
     if constexpr (LP) return reinterpret_cast<T*>(ps_dyn)[r];
     else return b.gpiv[r];
 }
+// PL: the block's panel also lives in LDS (m x 20, ld m) behind the running
+// values and the permutation, for blocks whose panel fits (m <= ~940 rows in
+// FP64): its reads are then LDS reads, the copy in Y is for the scatter
+template <typename T>
+__device__ __forceinline__ T* lz_pan(int64_t m) {
+    return reinterpret_cast<T*>(ps_dyn + ((size_t)m * (sizeof(T) + sizeof(int)) + 15) / 16 * 16);
+}
 template <typename T, bool LP>
 __device__ __forceinline__ int& lz_perm(const PsLazy<T>& b, int64_t m, int64_t r) {
     if constexpr (LP) return reinterpret_cast<int*>(reinterpret_cast<T*>(ps_dyn) + m)[r];
@@ -185,7 +192,7 @@ __device__ __forceinline__ int& lz_perm(const PsLazy<T>& b, int64_t m, int64_t r
 // panel and the permutation are exchanged after a barrier (every wavefront
 // has read them); a second barrier makes them visible to the next column.
 // Returns false when the factorisation stops at this column.
-template <typename T, bool LP, int R>
+template <typename T, bool LP, int R, bool PL>
 __device__ __forceinline__ bool pstrf_column(const PsLazy<T>& b, int j, int tid) {
     const int64_t n = b.n, k = b.k, m = n - k, c = k + j;
     const int lane = tid & 63, wave = tid >> 6;
@@ -197,17 +204,25 @@ __device__ __forceinline__ bool pstrf_column(const PsLazy<T>& b, int j, int tid)
     const bool sw = p != c;
     const int64_t pl = p - k;  // local index of the pivot row
     T* const Y = b.Y;
+    T* const pan = PL ? lz_pan<T>(m) : nullptr;
+    // panel entry (local row r, column s): read from LDS when it is there;
+    // written to Y (and LDS)
+    auto prd = [&](int64_t r, int s) -> T { return PL ? pan[r + (int64_t)s * m] : Y[r + (int64_t)s * n]; };
+    auto pwr = [&](int64_t r, int s, T x) {
+        Y[r + (int64_t)s * n] = x;
+        if constexpr (PL) pan[r + (int64_t)s * m] = x;
+    };
     T rp = T(0), rc = T(0);  // lane s < j: panel rows p and c before the swap
     if (lane < j) {
-        rp = Y[pl + lane * n];
-        rc = Y[j + lane * n];
+        rp = prd(pl, lane);
+        rc = prd(j, lane);
     }
     if ((double)pv < b.eps) {
         // the swap of rows c and p in the block's columns still applies; the
         // rest of the swap lies in the square that is cleared
         if (wave == 0 && sw && lane < j) {
-            Y[j + lane * n] = rp;
-            Y[pl + lane * n] = rc;
+            pwr(j, lane, rp);
+            pwr(pl, lane, rc);
         }
         if (tid == 0) {
             b.P[c] = p;
@@ -239,10 +254,10 @@ __device__ __forceinline__ bool pstrf_column(const PsLazy<T>& b, int j, int tid)
             const int64_t r = q0 + (int64_t)u * kPsThreads;
             if (r >= m) continue;
             const int64_t rl = r;
-            const T* row = Y + (sw && rl == pl ? (int64_t)j : rl);
+            const int64_t ri = sw && rl == pl ? (int64_t)j : rl;
             T a[kPsBlock - 1];
 #pragma unroll
-            for (int s = 0; s < kPsBlock - 1; ++s) a[s] = s < j ? row[s * n] : T(0);
+            for (int s = 0; s < kPsBlock - 1; ++s) a[s] = s < j ? prd(ri, s) : T(0);
 #pragma unroll
             for (int s = 0; s < kPsBlock - 1; ++s) {
                 if (s < j) acc[u] += a[s] * ps_lane(rp, s);
@@ -254,7 +269,7 @@ __device__ __forceinline__ bool pstrf_column(const PsLazy<T>& b, int j, int tid)
             const int64_t r = q0 + (int64_t)u * kPsThreads;
             if (!(r > j && r < m)) continue;
             const T y = (raw[u] - acc[u]) / d;
-            Y[r + j * n] = y;
+            pwr(r, j, y);
             const T x = lz_piv<T, LP>(b, sw && r == pl ? (int64_t)j : r) - y * y;
             lz_piv<T, LP>(b, r) = x;
             argmax_combine(nv, ni, x, (int)(k + r));
@@ -267,12 +282,12 @@ __device__ __forceinline__ bool pstrf_column(const PsLazy<T>& b, int j, int tid)
     }
     __syncthreads();
     if (wave == 0 && sw && lane < j) {
-        Y[j + lane * n] = rp;
-        Y[pl + lane * n] = rc;
+        pwr(j, lane, rp);
+        pwr(pl, lane, rc);
     }
     if (tid == 0) {
         b.P[c] = p;
-        Y[j + j * n] = d;
+        pwr(j, j, d);
         if (sw) {
             lz_perm<T, LP>(b, m, j) = phys_p;
             lz_perm<T, LP>(b, m, pl) = phys_c;
@@ -284,7 +299,7 @@ __device__ __forceinline__ bool pstrf_column(const PsLazy<T>& b, int j, int tid)
 
 // the columns k .. k+19 (fewer at the end) of the factorisation, one
 // workgroup; the running values and the permutation live in LDS when LP
-template <typename T, bool LP, int R>
+template <typename T, bool LP, int R, bool PL>
 __global__ __launch_bounds__(kPsThreads) void pstrf_block_kernel(PsLazy<T> b) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (*b.done != 0) {
@@ -312,7 +327,7 @@ __global__ __launch_bounds__(kPsThreads) void pstrf_block_kernel(PsLazy<T> b) {
     __syncthreads();
     const int cur = (int)min((int64_t)kPsBlock, m);
     for (int j = 0; j < cur; ++j)
-        if (!pstrf_column<T, LP, R>(b, j, tid)) return;
+        if (!pstrf_column<T, LP, R, PL>(b, j, tid)) return;
     // the permutation of the trailing rows, and the rows it moves
     for (int64_t r = cur + tid; r < m; r += kPsThreads) {
         const int pr = lz_perm<T, LP>(b, m, r);
@@ -519,22 +534,29 @@ rmh_status pstrf_impl(Context* h, bool upper, int64_t n, T* A, int64_t lda, int6
     hipStream_t side = nullptr;
     hipEvent_t ev_block = nullptr, ev_swaps = nullptr;
     if (n > kPsBlock) RMH_TRY(h->side_stream(&side, &ev_block, &ev_swaps));
-    constexpr size_t kLdsBudget = 96 * 1024;
+    constexpr size_t kLdsBudget = 96 * 1024;        // running values and permutation
+    constexpr size_t kLdsPanelBudget = 156 * 1024;  // with the panel as well
     for (int64_t k = 0; k < n; k += kPsBlock) {
         const int64_t m = n - k;
         const int cur = (int)std::min<int64_t>(kPsBlock, m);
         const bool lp = (size_t)m * (sizeof(T) + sizeof(int)) <= kLdsBudget;
         const PsLazy<T> b{A, lda, n, k, Y, gpiv, gperm, piv_out, rank, done, info, eps, 0.0};
         const size_t dyn = lp ? (size_t)m * (sizeof(T) + sizeof(int)) : 0;
+        const size_t dyn_pan = (dyn + 15) / 16 * 16 + (size_t)m * kPsBlock * sizeof(T);
+        const bool pl = lp && dyn_pan <= kLdsPanelBudget;
         const bool four = sizeof(T) == 4 || m > 2 * kPsThreads;
-        if (lp && four)
-            hipLaunchKernelGGL((pstrf_block_kernel<T, true, kPsRowsMax>), dim3(1), dim3(kPsThreads), dyn, s, b);
+        if (pl && four)
+            hipLaunchKernelGGL((pstrf_block_kernel<T, true, kPsRowsMax, true>), dim3(1), dim3(kPsThreads), dyn_pan, s, b);
+        else if (pl)
+            hipLaunchKernelGGL((pstrf_block_kernel<T, true, 2, true>), dim3(1), dim3(kPsThreads), dyn_pan, s, b);
+        else if (lp && four)
+            hipLaunchKernelGGL((pstrf_block_kernel<T, true, kPsRowsMax, false>), dim3(1), dim3(kPsThreads), dyn, s, b);
         else if (lp)
-            hipLaunchKernelGGL((pstrf_block_kernel<T, true, 2>), dim3(1), dim3(kPsThreads), dyn, s, b);
+            hipLaunchKernelGGL((pstrf_block_kernel<T, true, 2, false>), dim3(1), dim3(kPsThreads), dyn, s, b);
         else if (four)
-            hipLaunchKernelGGL((pstrf_block_kernel<T, false, kPsRowsMax>), dim3(1), dim3(kPsThreads), 0, s, b);
+            hipLaunchKernelGGL((pstrf_block_kernel<T, false, kPsRowsMax, false>), dim3(1), dim3(kPsThreads), 0, s, b);
         else
-            hipLaunchKernelGGL((pstrf_block_kernel<T, false, 2>), dim3(1), dim3(kPsThreads), 0, s, b);
+            hipLaunchKernelGGL((pstrf_block_kernel<T, false, 2, false>), dim3(1), dim3(kPsThreads), 0, s, b);
         RMH_TRY(check_launch("pstrf_block_kernel"));
         const int64_t h2 = m - cur;
         if (h2 > 0) {
